@@ -1,0 +1,231 @@
+/*
+ * smalgo.h — C ABI of the MI355X (gfx950) solve-free interior eigensolver.
+ *
+ * Drop-in boundary for the two solve-free drivers of
+ * MathiasTensor/sparse_matrix_algorithms:
+ *   chebyshev_poly_real_symm_sparse  (chebyshev_subspace_iteration.jl:667-757, :771-848)
+ *   polfed                           (polfed.jl:1128-1158, :1183-1186)
+ * and of the helpers those drivers call (bounds, coefficients, the block
+ * Clenshaw filter).  A Julia module `ccall`s these symbols with the
+ * reference's own argument meaning; a Python ctypes mirror does the same.
+ *
+ * Conventions
+ *   - Host arrays are column-major (Julia layout), FP64, owned by the caller.
+ *   - The library owns all device memory (per context / matrix / result).
+ *   - Every call returns an int status: SMALGO_OK (0) or a negative code;
+ *     smalgo_last_error(ctx) gives the message of the last failure on ctx.
+ *   - One context per host thread; callbacks run on the calling thread only.
+ */
+#ifndef SMALGO_H
+#define SMALGO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMALGO_OK 0
+#define SMALGO_ERR_ARG -1       /* ArgumentError / error(...) in the reference */
+#define SMALGO_ERR_HIP -2       /* HIP runtime failure */
+#define SMALGO_ERR_CONFIG -3    /* unsupported tile / configuration */
+#define SMALGO_ERR_INTERNAL -4  /* workspace or library failure */
+#define SMALGO_ERR_CALLBACK -5  /* user matblock callback returned nonzero */
+
+/* window_type (polfed.jl:404-414 `window_type` Symbol) */
+#define SMALGO_WINDOW_STEP 0
+#define SMALGO_WINDOW_DELTA 1
+#define SMALGO_WINDOW_SMOOTH 2
+
+/* precision of the filter / basis (north-star mixed mode; FP64 = reference) */
+#define SMALGO_PREC_FP64 0
+#define SMALGO_PREC_FP32_REFINE64 1
+
+typedef struct smalgo_ctx smalgo_ctx;
+typedef struct smalgo_mat smalgo_mat;
+typedef struct smalgo_result smalgo_result;
+
+/* User block operator Y <- A*X on host arrays (n x b, column-major, leading
+ * dimensions ldx/ldy).  Replaces MatrixFreeRealSymOp.matblock!
+ * (polfed.jl:148-165, chebyshev_subspace_iteration.jl:151-167) and, through a
+ * column loop, matvec!.  Returns 0 on success. */
+typedef int (*smalgo_matblock_fn)(void* user, int64_t n, int64_t b, const double* X, int64_t ldx,
+                                  double* Y, int64_t ldy);
+
+/* ---- context ---------------------------------------------------------- */
+int smalgo_ctx_create(int device, smalgo_ctx** out);
+void smalgo_ctx_destroy(smalgo_ctx* ctx);
+const char* smalgo_last_error(const smalgo_ctx* ctx);
+/* Library version string, e.g. "smalgo 0.1 gfx950". */
+const char* smalgo_version(void);
+
+/* ---- sparse matrix ------------------------------------------------------
+ * Upload a compressed sparse matrix.  ptr has n+1 entries, idx/val have nnz.
+ * For the symmetric matrices of this path, Julia's SparseMatrixCSC
+ * (colptr,rowval,nzval) of A is the CSR of A, so both layouts are accepted
+ * as CSR.  index_base is 1 for Julia arrays, 0 for C/Python.
+ * Replaces the SparseMatrixCSC{Float64,Int} operand of apply_A_block!
+ * (polfed.jl:196-199) and mul!(AX,A,B1) (chebyshev_subspace_iteration.jl:574). */
+int smalgo_csr_upload(smalgo_ctx* ctx, int64_t n, const int64_t* ptr, const int64_t* idx,
+                      const double* val, int index_base, smalgo_mat** out);
+void smalgo_mat_free(smalgo_mat* mat);
+int64_t smalgo_mat_n(const smalgo_mat* mat);
+int64_t smalgo_mat_nnz(const smalgo_mat* mat);
+/* exact symmetry test of the uploaded pattern and values
+ * (issymmetric, chebyshev_subspace_iteration.jl:669) */
+int smalgo_mat_is_symmetric(const smalgo_mat* mat, int* is_symmetric);
+
+/* gershgorin_bounds (polfed.jl:254-266, chebyshev_subspace_iteration.jl:351-367) */
+int smalgo_gershgorin_bounds(const smalgo_mat* mat, double* smin, double* smax);
+
+/* ---- filter coefficients (host) -----------------------------------------
+ * jackson_kernel (polfed.jl:275-284): g[0..K] (K<=0 gives g[0]=1). */
+int smalgo_jackson_kernel(int K, double* g);
+/* cheb_delta_coeffs (polfed.jl:299-313): c[0..K] */
+int smalgo_cheb_delta_coeffs(double sigma_hat, int K, int jackson, double* c);
+/* cheb_window_coeffs (polfed.jl:329-347): c[0..K] */
+int smalgo_cheb_window_coeffs(double a, double b, int K, int jackson, double* c);
+/* cheb_smooth_window_coeffs (polfed.jl:364-388): c[0..K], nsamp<=0 -> 4096, delta<=0 -> 0.02 */
+int smalgo_cheb_smooth_window_coeffs(double a, double b, int K, int jackson, int nsamp,
+                                     double delta, double* c);
+/* _build_coeffs (polfed.jl:404-414): maps sigma/emin/emax by c=(smin+smax)/2,
+ * d=(smax-smin)/2 and dispatches on window_type. */
+int smalgo_build_coeffs(int degree, double smin, double smax, double sigma, double emin,
+                        double emax, int window_type, int jackson, double* c);
+
+/* ---- block Clenshaw filter ----------------------------------------------
+ * Y = p(Â) Q with Â=(A-cI)/d, p = sum_k coeffs[k] T_k
+ * (apply_poly_block!, polfed.jl:464-501 / chebyshev_subspace_iteration.jl:554-591).
+ * Q, Y: host n x b column-major.  precision: SMALGO_PREC_FP64 or
+ * SMALGO_PREC_FP32_REFINE64 (filter in FP32, result returned as FP64). */
+int smalgo_apply_poly_block(smalgo_ctx* ctx, const smalgo_mat* A, int64_t b, const double* Q,
+                            int64_t ldq, const double* coeffs, int K, double smin, double smax,
+                            int precision, double* Y, int64_t ldy);
+/* Same with a host operator (matrix-free form, polfed.jl:508-545). */
+int smalgo_apply_poly_block_mf(smalgo_ctx* ctx, int64_t n, smalgo_matblock_fn fn, void* user,
+                               int64_t b, const double* Q, int64_t ldq, const double* coeffs,
+                               int K, double smin, double smax, double* Y, int64_t ldy);
+
+/* ---- block kernels of the reference, one call each ----------------------
+ * All blocks are host arrays, n x b column-major; precision as above (the
+ * FP32 mode rounds the inputs to FP32 and computes the products in FP32).
+ *
+ * orth_against!(W, Q): W <- W - Q (Qᵀ W)   (polfed.jl:605-609) */
+int smalgo_orth_against(smalgo_ctx* ctx, int64_t n, int64_t b, double* W, int64_t ldw, int64_t m,
+                        const double* Q, int64_t ldq, int precision);
+/* block_qr_factor(W; rtol) / orth_block_polfed!(W; rtol) (polfed.jl:644-685,
+ * :561-594): W = Q R by CholQR2 (shifted CholQR3 when the Gram matrix is not
+ * numerically positive definite); rank = #{|R_ii| > rtol·max|R_ii|}.  Q gets
+ * all b columns (the caller keeps the first rank), R is b x b upper
+ * triangular, column-major (ldr >= b). */
+int smalgo_block_qr(smalgo_ctx* ctx, int64_t n, int64_t b, const double* W, int64_t ldw,
+                    double rtol, int precision, double* Q, int64_t ldq, double* R, int64_t ldr,
+                    int* rank);
+/* build_X!(X, Qhist, U): X = V U with V = [Q_1 … Q_m] as one n x M block
+ * (polfed.jl:767-776); U is M x k column-major. */
+int smalgo_build_X(smalgo_ctx* ctx, int64_t n, int64_t M, int64_t k, const double* V, int64_t ldv,
+                   const double* U, int64_t ldu, double* X, int64_t ldx);
+/* rayleigh!(λ, res, AX, A, X) (polfed.jl:791-830): λ_j = x_jᵀ A x_j,
+ * res_j = ‖A x_j − λ_j x_j‖₂.  Exactly one of A or fn. */
+int smalgo_rayleigh(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn, void* user,
+                    int64_t n, int64_t k, const double* X, int64_t ldx, double* lam, double* res);
+/* residuals!(res, A, X, λ, AX) (chebyshev_subspace_iteration.jl:311-344) */
+int smalgo_residuals(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn, void* user,
+                     int64_t n, int64_t k, const double* X, int64_t ldx, const double* lam,
+                     double* res);
+/* refine_original_rr(A, X) (polfed.jl:837-880): result holds (λ, Y, res)
+ * sorted by λ; read it with smalgo_result_*. */
+int smalgo_refine_original_rr(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn,
+                              void* user, int64_t n, int64_t k, const double* X, int64_t ldx,
+                              smalgo_result** out);
+
+/* ---- POLFED ---------------------------------------------------------------
+ * Keyword arguments of polfed(A::MatrixFreeRealSymOp,nev,degree;...)
+ * (polfed.jl:1128) plus the mixed-precision controls.  has_* flags stand in
+ * for Julia's `nothing` defaults. */
+typedef struct smalgo_polfed_params {
+  int s;                 /* block size (default 8) */
+  int has_bounds;        /* smin/smax given; otherwise Gershgorin (sparse form only) */
+  double smin, smax;
+  double sigma;          /* default 0.0 */
+  double emin, emax;     /* default -Inf, +Inf */
+  double tol;            /* default 1e-10 */
+  int overextract;       /* default 2 */
+  int refine;            /* default 1 */
+  int window_type;       /* default SMALGO_WINDOW_STEP */
+  int jackson;           /* default 1 */
+  int mmax;              /* <=0: ceil(2.8*nev/s) */
+  double rtol;           /* default 1e-12 */
+  int reorth_passes;     /* default 1 */
+  int extra_keep;        /* default 8 */
+  uint64_t seed;         /* default 1 (start block when Q1 == NULL) */
+  int chunk;             /* default 64 (kept for signature parity; the device lifts all at once) */
+  int show_progress;     /* default 0: print the progress line (polfed.jl:991-1000) */
+  /* mixed precision (north star; no reference counterpart) */
+  int precision;         /* SMALGO_PREC_* (default FP64) */
+  int refine_guard;      /* guard vectors in FP64 refinement (default -1: nev/2) */
+  int refine_degree;     /* degree of the FP64 refinement filter (default -1: degree) */
+  int refine_maxiter;    /* FP64 refinement iterations with locking (default 3) */
+  double refine_tol;     /* relative residual target for locking (default 1e-10 * 0.5) */
+} smalgo_polfed_params;
+
+void smalgo_polfed_default_params(smalgo_polfed_params* p);
+
+/* polfed(A, nev, degree; kwargs...).  Q1 (optional, n x s column-major) is the
+ * start block BEFORE orthonormalisation — the reference's randn(n,s) after
+ * Random.seed!(seed) (polfed.jl:1133-1134); NULL draws it from the library's
+ * splitmix64/Box-Muller stream with p->seed.  Exactly one of A or fn is set
+ * (fn: matrix-free form, polfed.jl:1183-1186, FP64 only). */
+int smalgo_polfed(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn, void* user,
+                  int64_t n, int nev, int degree, const smalgo_polfed_params* p,
+                  const double* Q1, smalgo_result** out);
+
+/* ---- Chebyshev filtered subspace iteration ------------------------------ */
+typedef struct smalgo_cheb_params {
+  int has_bounds;
+  double smin, smax;
+  double sigma;      /* σ, default 0.0 */
+  int m0;            /* default 200 */
+  int degree;        /* default 80 */
+  int maxiter;       /* default 8 */
+  double tol;        /* default 1e-10 */
+  double res_gate;   /* default 1e-6 */
+  int jackson;       /* default 1 */
+  int debug;         /* default 0 */
+  int two_hit;       /* default 1 */
+  uint64_t seed;     /* default 1 */
+  int window_type;   /* default SMALGO_WINDOW_STEP */
+} smalgo_cheb_params;
+
+void smalgo_cheb_default_params(smalgo_cheb_params* p);
+
+/* chebyshev_poly_real_symm_sparse(A, emin, emax; ...) (chebyshev_subspace_iteration.jl:667-757):
+ * always runs maxiter iterations.  With fn set instead of A it is the
+ * matvec! form (:771-848): bounds required, two_hit early exit.  Q0
+ * (optional, n x m0 column-major) replaces randn(n,m0) (:701-702). */
+int smalgo_chebyshev(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn, void* user,
+                     int64_t n, double emin, double emax, const smalgo_cheb_params* p,
+                     const double* Q0, smalgo_result** out);
+
+/* ---- results ---------------------------------------------------------------
+ * (λ::Vector, X::Matrix n x k column-major, res::Vector) as returned by the
+ * drivers (polfed.jl:1148-1157, chebyshev_subspace_iteration.jl:750-756). */
+int64_t smalgo_result_count(const smalgo_result* r);
+int64_t smalgo_result_n(const smalgo_result* r);
+/* any of lam / X / res may be NULL to skip it */
+int smalgo_result_copy(const smalgo_result* r, double* lam, double* X, int64_t ldx, double* res);
+/* named run statistics (seconds, counts); returns SMALGO_ERR_ARG for unknown names.
+ * Names: "t_total", "t_filter", "t_orth", "t_extract", "t_refine", "filter_steps",
+ * "block_steps", "basis_dim", "iterations", "spmm_bytes", "gemm_flops",
+ * "refine_filter_steps", "locked". */
+int smalgo_result_stat(const smalgo_result* r, const char* name, double* value);
+/* the progress line of polfed.jl:991-1000 for this result */
+const char* smalgo_result_progress_line(const smalgo_result* r);
+void smalgo_result_free(smalgo_result* r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SMALGO_H */

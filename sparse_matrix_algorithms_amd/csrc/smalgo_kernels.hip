@@ -1,0 +1,826 @@
+// HIP kernels for gfx950 (MI355X): fused CSR Clenshaw step, MFMA tall-skinny
+// GEMMs, column reductions and small block helpers.
+//
+// Block layout: row-major n x b blocks (element (i,c) at i*ld + c), so the
+// b values of one row are contiguous and the CSR gather A·X reads whole rows
+// of X with coalesced vector loads.
+#include "smalgo_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace smalgo {
+
+// ---------------------------------------------------------------------------
+// vector load/store helpers
+// ---------------------------------------------------------------------------
+template <typename T, int V> struct VecT;
+template <typename T> struct VecT<T, 1> { using type = T; };
+template <> struct VecT<float, 2> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct VecT<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct VecT<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct VecT<double, 4> { typedef double type __attribute__((ext_vector_type(4))); };
+
+template <typename T, int V>
+__device__ __forceinline__ void ldv(const T* __restrict__ p, T (&r)[V]) {
+  if constexpr (V == 1) {
+    r[0] = *p;
+  } else {
+    using VT = typename VecT<T, V>::type;
+    VT v = *reinterpret_cast<const VT*>(p);
+#pragma unroll
+    for (int e = 0; e < V; ++e) r[e] = v[e];
+  }
+}
+template <typename T, int V>
+__device__ __forceinline__ void ldv_alias(const T* p, T (&r)[V]) {
+  if constexpr (V == 1) {
+    r[0] = *p;
+  } else {
+    using VT = typename VecT<T, V>::type;
+    VT v = *reinterpret_cast<const VT*>(p);
+#pragma unroll
+    for (int e = 0; e < V; ++e) r[e] = v[e];
+  }
+}
+template <typename T, int V>
+__device__ __forceinline__ void stv(T* p, const T (&r)[V]) {
+  if constexpr (V == 1) {
+    *p = r[0];
+  } else {
+    using VT = typename VecT<T, V>::type;
+    VT v;
+#pragma unroll
+    for (int e = 0; e < V; ++e) v[e] = r[e];
+    *reinterpret_cast<VT*>(p) = v;
+  }
+}
+
+// XCD-aware block index: the dispatcher deals workgroups round-robin over the
+// 8 XCDs (workgroup w lands on XCD w % 8).  Mapping logical tile
+// (w % 8) * per_xcd + w / 8 gives each XCD one contiguous range of rows, so
+// the rows it gathers for its CSR neighbours stay in its own L2.
+__device__ __forceinline__ int xcd_tile(int per_xcd) {
+  const int w = blockIdx.x;
+  return (w & 7) * per_xcd + (w >> 3);
+}
+
+static inline int grid_xcd(int ntiles, int* per_xcd) {
+  const int g = (ntiles + 7) / 8 * 8;
+  *per_xcd = g / 8;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// fused Clenshaw step
+//   out = cq*Q + a1*(A X1 or P) + b1*X1 - s2*X2
+// One row per team of LPR lanes; each lane covers V consecutive columns and
+// strides by LPR*V over the block width.  64/LPR rows per wavefront, four
+// wavefronts per workgroup.
+// ---------------------------------------------------------------------------
+template <typename T, int V, int LPR, bool PERCOL>
+__global__ __launch_bounds__(256) void clenshaw_kernel(
+    int n, int b, int ntiles, int per_xcd, const int* __restrict__ rowptr,
+    const int* __restrict__ colidx, const T* __restrict__ val, const T* __restrict__ Q, int ldq,
+    const T* __restrict__ X1, int ld1, const T* __restrict__ P, int ldp, const T* X2, int ld2,
+    T* out, int ldo, T cq, T a1, T b1, T s2, const T* __restrict__ pcq,
+    const T* __restrict__ pa1, const T* __restrict__ pb1, const T* __restrict__ ps2) {
+  constexpr int RPW = 64 / LPR;
+  constexpr int RPB = 4 * RPW;
+  const int tile = xcd_tile(per_xcd);
+  if (tile >= ntiles) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int row = tile * RPB + wave * RPW + lane / LPR;
+  if (row >= n) return;
+  const int tl = lane % LPR;
+  const bool gather = (X1 != nullptr) && (P == nullptr);
+  int p0 = 0, p1 = 0;
+  if (gather) {
+    p0 = rowptr[row];
+    p1 = rowptr[row + 1];
+  }
+  for (int c = tl * V; c < b; c += LPR * V) {
+    T acc[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) acc[e] = T(0);
+    const bool full = (c + V <= b);
+    if (full) {
+      if (gather) {
+        int p = p0;
+        for (; p + 4 <= p1; p += 4) {
+          const int j0 = colidx[p], j1 = colidx[p + 1], j2 = colidx[p + 2], j3 = colidx[p + 3];
+          const T v0 = val[p], v1 = val[p + 1], v2 = val[p + 2], v3 = val[p + 3];
+          T x0[V], x1[V], x2[V], x3[V];
+          ldv<T, V>(X1 + (size_t)j0 * ld1 + c, x0);
+          ldv<T, V>(X1 + (size_t)j1 * ld1 + c, x1);
+          ldv<T, V>(X1 + (size_t)j2 * ld1 + c, x2);
+          ldv<T, V>(X1 + (size_t)j3 * ld1 + c, x3);
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            acc[e] += v0 * x0[e];
+            acc[e] += v1 * x1[e];
+            acc[e] += v2 * x2[e];
+            acc[e] += v3 * x3[e];
+          }
+        }
+        for (; p < p1; ++p) {
+          const int j0 = colidx[p];
+          const T v0 = val[p];
+          T x0[V];
+          ldv<T, V>(X1 + (size_t)j0 * ld1 + c, x0);
+#pragma unroll
+          for (int e = 0; e < V; ++e) acc[e] += v0 * x0[e];
+        }
+      } else if (P != nullptr) {
+        ldv<T, V>(P + (size_t)row * ldp + c, acc);
+      }
+      T r[V];
+#pragma unroll
+      for (int e = 0; e < V; ++e) r[e] = (PERCOL ? pa1[c + e] : a1) * acc[e];
+      if (Q != nullptr) {
+        T q[V];
+        ldv<T, V>(Q + (size_t)row * ldq + c, q);
+#pragma unroll
+        for (int e = 0; e < V; ++e) r[e] = (PERCOL ? pcq[c + e] : cq) * q[e] + r[e];
+      }
+      if (X1 != nullptr) {
+        T x[V];
+        ldv<T, V>(X1 + (size_t)row * ld1 + c, x);
+#pragma unroll
+        for (int e = 0; e < V; ++e) r[e] += (PERCOL ? pb1[c + e] : b1) * x[e];
+      }
+      if (X2 != nullptr) {
+        T y[V];
+        ldv_alias<T, V>(X2 + (size_t)row * ld2 + c, y);
+#pragma unroll
+        for (int e = 0; e < V; ++e) r[e] -= (PERCOL ? ps2[c + e] : s2) * y[e];
+      }
+      stv<T, V>(out + (size_t)row * ldo + c, r);
+    } else {
+      // ragged tail of the row: element-wise
+      for (int e = 0; e < V && c + e < b; ++e) {
+        const int cc = c + e;
+        T a = T(0);
+        if (gather) {
+          for (int p = p0; p < p1; ++p) a += val[p] * X1[(size_t)colidx[p] * ld1 + cc];
+        } else if (P != nullptr) {
+          a = P[(size_t)row * ldp + cc];
+        }
+        T r = (PERCOL ? pa1[cc] : a1) * a;
+        if (Q != nullptr) r = (PERCOL ? pcq[cc] : cq) * Q[(size_t)row * ldq + cc] + r;
+        if (X1 != nullptr) r += (PERCOL ? pb1[cc] : b1) * X1[(size_t)row * ld1 + cc];
+        if (X2 != nullptr) r -= (PERCOL ? ps2[cc] : s2) * X2[(size_t)row * ld2 + cc];
+        out[(size_t)row * ldo + cc] = r;
+      }
+    }
+  }
+}
+
+template <typename T, int V, int LPR, bool PERCOL>
+static void clenshaw_launch_t(hipStream_t st, const DevCSR& A, int b, const T* Q, int ldq,
+                              const T* X1, int ld1, const T* P, int ldp, const T* X2, int ld2,
+                              T* out, int ldo, T cq, T a1, T b1, T s2, const T* pcq, const T* pa1,
+                              const T* pb1, const T* ps2) {
+  constexpr int RPB = 4 * (64 / LPR);
+  const int ntiles = (A.n + RPB - 1) / RPB;
+  int per_xcd = 0;
+  const int grid = grid_xcd(ntiles, &per_xcd);
+  hipLaunchKernelGGL((clenshaw_kernel<T, V, LPR, PERCOL>), dim3(grid), dim3(256), 0, st, A.n, b,
+                     ntiles, per_xcd, A.rowptr, A.col, A.vals<T>(), Q, ldq, X1, ld1, P, ldp, X2,
+                     ld2, out, ldo, cq, a1, b1, s2, pcq, pa1, pb1, ps2);
+  SMALGO_HIP(hipGetLastError());
+}
+
+template <typename T, bool PERCOL>
+static void clenshaw_dispatch(hipStream_t st, const DevCSR& A, int b, const T* Q, int ldq,
+                              const T* X1, int ld1, const T* P, int ldp, const T* X2, int ld2,
+                              T* out, int ldo, T cq, T a1, T b1, T s2, const T* pcq, const T* pa1,
+                              const T* pb1, const T* ps2, FilterTile t) {
+#define SMALGO_CL(VV, LL)                                                                    \
+  if (t.vec == VV && t.lpr == LL)                                                            \
+    return clenshaw_launch_t<T, VV, LL, PERCOL>(st, A, b, Q, ldq, X1, ld1, P, ldp, X2, ld2,  \
+                                                out, ldo, cq, a1, b1, s2, pcq, pa1, pb1, ps2);
+#define SMALGO_CL_V(VV) \
+  SMALGO_CL(VV, 2) SMALGO_CL(VV, 4) SMALGO_CL(VV, 8) SMALGO_CL(VV, 16) SMALGO_CL(VV, 32) SMALGO_CL(VV, 64)
+  SMALGO_CL_V(1) SMALGO_CL_V(2) SMALGO_CL_V(4)
+#undef SMALGO_CL_V
+#undef SMALGO_CL
+  throw Error(-3, "unsupported filter tile vec=" + std::to_string(t.vec) +
+                      " lpr=" + std::to_string(t.lpr));
+}
+
+static void check_tile_alignment(FilterTile t, size_t elem, std::initializer_list<const void*> ptrs,
+                                 std::initializer_list<int> lds) {
+  const size_t al = elem * (size_t)t.vec;
+  for (const void* p : ptrs)
+    if (p && (reinterpret_cast<uintptr_t>(p) % al) != 0)
+      throw Error(-3, "filter tile vec=" + std::to_string(t.vec) + " needs aligned block pointers");
+  for (int ld : lds)
+    if (ld % t.vec != 0)
+      throw Error(-3, "filter tile vec=" + std::to_string(t.vec) + " needs ld % vec == 0");
+}
+
+template <typename T>
+void launch_clenshaw_step(hipStream_t st, const DevCSR& A, int b, const T* Q, int ldq, T cq,
+                          const T* X1, int ld1, T a1, T b1, const T* X2, int ld2, T s2, T* out,
+                          int ldo, FilterTile tile) {
+  if (A.n == 0 || b == 0) return;
+  check_tile_alignment(tile, sizeof(T), {Q, X1, X2, out}, {Q ? ldq : 0, X1 ? ld1 : 0, X2 ? ld2 : 0, ldo});
+  clenshaw_dispatch<T, false>(st, A, b, Q, ldq, X1, ld1, nullptr, 0, X2, ld2, out, ldo, cq, a1, b1,
+                              s2, nullptr, nullptr, nullptr, nullptr, tile);
+}
+
+template <typename T>
+void launch_clenshaw_step_cols(hipStream_t st, const DevCSR& A, int b, const T* Q, int ldq,
+                               const T* cq, const T* X1, int ld1, const T* a1, const T* b1,
+                               const T* X2, int ld2, const T* s2, T* out, int ldo, FilterTile tile) {
+  if (A.n == 0 || b == 0) return;
+  check_tile_alignment(tile, sizeof(T), {Q, X1, X2, out}, {Q ? ldq : 0, X1 ? ld1 : 0, X2 ? ld2 : 0, ldo});
+  clenshaw_dispatch<T, true>(st, A, b, Q, ldq, X1, ld1, nullptr, 0, X2, ld2, out, ldo, T(0), T(0),
+                             T(0), T(0), cq, a1, b1, s2, tile);
+}
+
+// Axpby form with a precomputed operator product P = A·X1 (host-callback
+// operators): out = cq*Q + a1*P + b1*X1 - s2*X2.
+void launch_clenshaw_step_pre(hipStream_t st, int n, int b, const double* Q, int ldq, double cq,
+                              const double* P, int ldp, const double* X1, int ld1, double a1,
+                              double b1, const double* X2, int ld2, double s2, double* out,
+                              int ldo) {
+  if (n == 0 || b == 0) return;
+  DevCSR fake;
+  fake.n = n;
+  FilterTile t{1, 64};
+  clenshaw_dispatch<double, false>(st, fake, b, Q, ldq, X1, ld1, P, ldp, X2, ld2, out, ldo, cq, a1,
+                                   b1, s2, nullptr, nullptr, nullptr, nullptr, t);
+}
+
+template void launch_clenshaw_step<float>(hipStream_t, const DevCSR&, int, const float*, int, float,
+                                          const float*, int, float, float, const float*, int,
+                                          float, float*, int, FilterTile);
+template void launch_clenshaw_step<double>(hipStream_t, const DevCSR&, int, const double*, int,
+                                           double, const double*, int, double, double,
+                                           const double*, int, double, double*, int, FilterTile);
+template void launch_clenshaw_step_cols<float>(hipStream_t, const DevCSR&, int, const float*, int,
+                                               const float*, const float*, int, const float*,
+                                               const float*, const float*, int, const float*,
+                                               float*, int, FilterTile);
+template void launch_clenshaw_step_cols<double>(hipStream_t, const DevCSR&, int, const double*, int,
+                                                const double*, const double*, int, const double*,
+                                                const double*, const double*, int, const double*,
+                                                double*, int, FilterTile);
+
+// ---------------------------------------------------------------------------
+// MFMA tall-skinny GEMMs (16x16x4 tiles; 4 wavefronts, each a 32x32 sub-tile
+// of a 64x64 workgroup tile; K staged through LDS 16 at a time).
+//
+// v_mfma_*_16x16x4 operand layout used below: lane l supplies A[l%16][l/16]
+// and B[l/16][l%16].  Accumulator element r of lane l is D[row][l%16] with
+// row = 4*(l/16)+r for f32 and row = l/16 + 4*r for f64 (measured on gfx950).
+// ---------------------------------------------------------------------------
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+template <typename TM> struct Mfma;
+template <> struct Mfma<double> {
+  using acc_t = d4_t;
+  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int drow(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+  using acc_t = f4_t;
+  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <typename TA, typename TB, typename TC>
+using mfma_t = typename std::conditional<std::is_same<TA, double>::value ||
+                                             std::is_same<TB, double>::value ||
+                                             std::is_same<TC, double>::value,
+                                         double, float>::type;
+
+constexpr int GKT = 16;  // K depth staged per LDS round
+
+// C_part[z][i][j] = sum_{k in chunk z} A[k][i] * B[k][j]
+template <typename TA, typename TB, typename TM>
+__global__ __launch_bounds__(256) void gemm_tn_kernel(int n, int m, int b, int kchunk,
+                                                      const TA* __restrict__ A, int lda,
+                                                      const TB* __restrict__ B, int ldb,
+                                                      TM* __restrict__ part) {
+  __shared__ TM As[GKT][64 + 4];
+  __shared__ TM Bs[GKT][64 + 4];
+  using acc_t = typename Mfma<TM>::acc_t;
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64, z = blockIdx.z;
+  const int kb = z * kchunk;
+  const int ke = min(n, kb + kchunk);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
+  for (int k0 = kb; k0 < ke; k0 += GKT) {
+#pragma unroll
+    for (int e = tid; e < GKT * 64; e += 256) {
+      const int kk = e >> 6, ii = e & 63;
+      const int k = k0 + kk;
+      As[kk][ii] = (k < ke && i0 + ii < m) ? (TM)A[(size_t)k * lda + i0 + ii] : TM(0);
+      Bs[kk][ii] = (k < ke && j0 + ii < b) ? (TM)B[(size_t)k * ldb + j0 + ii] : TM(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GKT; kk += 4) {
+      const int kr = kk + (lane >> 4), cl = lane & 15;
+      const TM a0 = As[kr][wi + cl], a1 = As[kr][wi + 16 + cl];
+      const TM b0 = Bs[kr][wj + cl], b1 = Bs[kr][wj + 16 + cl];
+      acc[0][0] = Mfma<TM>::op(a0, b0, acc[0][0]);
+      acc[0][1] = Mfma<TM>::op(a0, b1, acc[0][1]);
+      acc[1][0] = Mfma<TM>::op(a1, b0, acc[1][0]);
+      acc[1][1] = Mfma<TM>::op(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
+  }
+  TM* P = part + (size_t)z * m * b;
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + wi + ti * 16 + Mfma<TM>::drow(lane, r);
+        const int j = j0 + wj + tj * 16 + (lane & 15);
+        if (i < m && j < b) P[(size_t)i * b + j] = acc[ti][tj][r];
+      }
+}
+
+template <typename TM, typename TC>
+__global__ void gemm_tn_reduce(int m, int b, int nsplit, const TM* __restrict__ part, double alpha,
+                               double beta, TC* C, int ldc) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)m * b) return;
+  const int i = (int)(idx / b), j = (int)(idx % b);
+  double s = 0.0;
+  for (int z = 0; z < nsplit; ++z) s += (double)part[(size_t)z * m * b + idx];
+  double v = alpha * s;
+  if (beta != 0.0) v += beta * (double)C[(size_t)i * ldc + j];
+  C[(size_t)i * ldc + j] = (TC)v;
+}
+
+static int gemm_tn_nsplit(int n, int m, int b) {
+  const int tiles = ((m + 63) / 64) * ((b + 63) / 64);
+  const int want = std::max(1, (4 * device_cu_count() + tiles - 1) / tiles);
+  const int maxs = std::max(1, n / 256);
+  return std::max(1, std::min(want, maxs));
+}
+
+size_t gemm_tn_work_elems(int n, int m, int b) {
+  return (size_t)gemm_tn_nsplit(n, m, b) * (size_t)m * (size_t)b;
+}
+
+template <typename TA, typename TB, typename TC>
+void gemm_tn(hipStream_t st, int n, int m, int b, double alpha, const TA* A, int lda, const TB* B,
+             int ldb, double beta, TC* C, int ldc, double* work, size_t work_elems) {
+  if (m == 0 || b == 0) return;
+  using TM = mfma_t<TA, TB, TC>;
+  const int nsplit = gemm_tn_nsplit(n, m, b);
+  const size_t need = (size_t)nsplit * m * b;
+  if (need > work_elems) throw Error(-4, "gemm_tn workspace too small");
+  int kchunk = (n + nsplit - 1) / nsplit;
+  kchunk = (kchunk + GKT - 1) / GKT * GKT;
+  TM* part = reinterpret_cast<TM*>(work);
+  if (n > 0) {
+    dim3 grid((m + 63) / 64, (b + 63) / 64, nsplit);
+    hipLaunchKernelGGL((gemm_tn_kernel<TA, TB, TM>), grid, dim3(256), 0, st, n, m, b, kchunk, A,
+                       lda, B, ldb, part);
+    SMALGO_HIP(hipGetLastError());
+  } else {
+    SMALGO_HIP(hipMemsetAsync(part, 0, sizeof(TM) * (size_t)m * b, st));
+  }
+  const size_t tot = (size_t)m * b;
+  hipLaunchKernelGGL((gemm_tn_reduce<TM, TC>), dim3((tot + 255) / 256), dim3(256), 0, st, m, b,
+                     n > 0 ? nsplit : 1, part, alpha, beta, C, ldc);
+  SMALGO_HIP(hipGetLastError());
+}
+
+// C[i][j] = alpha * sum_k A[i][k] B(k, j) + beta * C[i][j]
+template <typename TA, typename TB, typename TC, typename TM, bool TRANSB>
+__global__ __launch_bounds__(256) void gemm_nn_kernel(int n, int m, int b, double alpha,
+                                                      const TA* __restrict__ A, int lda,
+                                                      const TB* __restrict__ B, int ldb,
+                                                      double beta, TC* C, int ldc) {
+  __shared__ TM As[64][GKT + 1];
+  __shared__ TM Bs[GKT][64 + 4];
+  using acc_t = typename Mfma<TM>::acc_t;
+  const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
+  acc_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
+  for (int k0 = 0; k0 < m; k0 += GKT) {
+#pragma unroll
+    for (int e = tid; e < 64 * GKT; e += 256) {
+      const int rr = e / GKT, kk = e % GKT;
+      const int r = r0 + rr, k = k0 + kk;
+      As[rr][kk] = (r < n && k < m) ? (TM)A[(size_t)r * lda + k] : TM(0);
+    }
+#pragma unroll
+    for (int e = tid; e < GKT * 64; e += 256) {
+      const int kk = e >> 6, cc = e & 63;
+      const int k = k0 + kk, c = c0 + cc;
+      TM v = TM(0);
+      if (k < m && c < b) v = TRANSB ? (TM)B[(size_t)c * ldb + k] : (TM)B[(size_t)k * ldb + c];
+      Bs[kk][cc] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GKT; kk += 4) {
+      const int kr = kk + (lane >> 4), cl = lane & 15;
+      const TM a0 = As[wi + cl][kr], a1 = As[wi + 16 + cl][kr];
+      const TM b0 = Bs[kr][wj + cl], b1 = Bs[kr][wj + 16 + cl];
+      acc[0][0] = Mfma<TM>::op(a0, b0, acc[0][0]);
+      acc[0][1] = Mfma<TM>::op(a0, b1, acc[0][1]);
+      acc[1][0] = Mfma<TM>::op(a1, b0, acc[1][0]);
+      acc[1][1] = Mfma<TM>::op(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = r0 + wi + ti * 16 + Mfma<TM>::drow(lane, r);
+        const int j = c0 + wj + tj * 16 + (lane & 15);
+        if (i < n && j < b) {
+          double v = alpha * (double)acc[ti][tj][r];
+          if (beta != 0.0) v += beta * (double)C[(size_t)i * ldc + j];
+          C[(size_t)i * ldc + j] = (TC)v;
+        }
+      }
+}
+
+template <typename TA, typename TB, typename TC>
+void gemm_nn(hipStream_t st, int n, int m, int b, double alpha, const TA* A, int lda, const TB* B,
+             int ldb, bool transB, double beta, TC* C, int ldc) {
+  if (n == 0 || b == 0) return;
+  using TM = mfma_t<TA, TB, TC>;
+  dim3 grid((n + 63) / 64, (b + 63) / 64);
+  if (transB)
+    hipLaunchKernelGGL((gemm_nn_kernel<TA, TB, TC, TM, true>), grid, dim3(256), 0, st, n, m, b,
+                       alpha, A, lda, B, ldb, beta, C, ldc);
+  else
+    hipLaunchKernelGGL((gemm_nn_kernel<TA, TB, TC, TM, false>), grid, dim3(256), 0, st, n, m, b,
+                       alpha, A, lda, B, ldb, beta, C, ldc);
+  SMALGO_HIP(hipGetLastError());
+}
+
+#define SMALGO_GEMM_INST(TA, TB, TC)                                                          \
+  template void gemm_tn<TA, TB, TC>(hipStream_t, int, int, int, double, const TA*, int,       \
+                                    const TB*, int, double, TC*, int, double*, size_t);       \
+  template void gemm_nn<TA, TB, TC>(hipStream_t, int, int, int, double, const TA*, int,       \
+                                    const TB*, int, bool, double, TC*, int);
+SMALGO_GEMM_INST(double, double, double)
+SMALGO_GEMM_INST(float, float, float)
+SMALGO_GEMM_INST(float, float, double)
+SMALGO_GEMM_INST(float, double, double)
+SMALGO_GEMM_INST(double, float, double)
+SMALGO_GEMM_INST(float, double, float)
+#undef SMALGO_GEMM_INST
+
+// ---------------------------------------------------------------------------
+// column reductions
+// ---------------------------------------------------------------------------
+static void colred_shape(int n, int b, int* nblk, int* rows_per_blk, int* nrt) {
+  const int nct = std::min(b, 256);
+  *nrt = std::max(1, 256 / std::max(nct, 1));
+  int want = 4 * device_cu_count();
+  int rpb = std::max(64, (n + want - 1) / want);
+  *rows_per_blk = rpb;
+  *nblk = std::max(1, (n + rpb - 1) / rpb);
+}
+
+size_t colred_work_elems(int n, int b) {
+  int nblk, rpb, nrt;
+  colred_shape(n, b, &nblk, &rpb, &nrt);
+  return (size_t)nblk * nrt * b;
+}
+
+// mode 0: sum x*y; mode 1: sum (y - lam*x)^2
+template <typename TX, int MODE>
+__global__ __launch_bounds__(256) void colred_kernel(int n, int b, int rows_per_blk, int nrt,
+                                                     const TX* __restrict__ X, int ldx,
+                                                     const TX* __restrict__ Y, int ldy,
+                                                     const double* __restrict__ lam,
+                                                     double* __restrict__ part) {
+  const int nct = min(b, 256);
+  const int tid = threadIdx.x;
+  const int rs = tid / nct;
+  if (rs >= nrt) return;
+  const int r0 = blockIdx.x * rows_per_blk;
+  const int r1 = min(n, r0 + rows_per_blk);
+  for (int c = tid % nct; c < b; c += nct) {
+    double s = 0.0;
+    const double l = MODE == 1 ? lam[c] : 0.0;
+    for (int i = r0 + rs; i < r1; i += nrt) {
+      const double x = (double)X[(size_t)i * ldx + c];
+      const double y = (double)Y[(size_t)i * ldy + c];
+      if (MODE == 0) {
+        s += x * y;
+      } else {
+        const double d = y - l * x;
+        s += d * d;
+      }
+    }
+    part[((size_t)blockIdx.x * nrt + rs) * b + c] = s;
+  }
+}
+
+template <int MODE>
+__global__ void colred_finish(int b, int nparts, const double* __restrict__ part, double* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= b) return;
+  double s = 0.0;
+  for (int p = 0; p < nparts; ++p) s += part[(size_t)p * b + c];
+  out[c] = MODE == 1 ? sqrt(s) : s;
+}
+
+template <typename TX, int MODE>
+static void colred(hipStream_t st, int n, int b, const TX* X, int ldx, const TX* Y, int ldy,
+                   const double* lam, double* out, double* work, size_t work_elems) {
+  if (b == 0) return;
+  int nblk, rpb, nrt;
+  colred_shape(n, b, &nblk, &rpb, &nrt);
+  if ((size_t)nblk * nrt * b > work_elems) throw Error(-4, "column reduction workspace too small");
+  hipLaunchKernelGGL((colred_kernel<TX, MODE>), dim3(nblk), dim3(256), 0, st, n, b, rpb, nrt, X,
+                     ldx, Y, ldy, lam, work);
+  SMALGO_HIP(hipGetLastError());
+  hipLaunchKernelGGL((colred_finish<MODE>), dim3((b + 255) / 256), dim3(256), 0, st, b, nblk * nrt,
+                     work, out);
+  SMALGO_HIP(hipGetLastError());
+}
+
+template <typename TX>
+void col_dots(hipStream_t st, int n, int b, const TX* X, int ldx, const TX* Y, int ldy, double* out,
+              double* work, size_t work_elems) {
+  colred<TX, 0>(st, n, b, X, ldx, Y, ldy, nullptr, out, work, work_elems);
+}
+template <typename TX>
+void col_resid(hipStream_t st, int n, int b, const TX* X, int ldx, const TX* AX, int ldax,
+               const double* lam, double* out, double* work, size_t work_elems) {
+  colred<TX, 1>(st, n, b, X, ldx, AX, ldax, lam, out, work, work_elems);
+}
+template void col_dots<double>(hipStream_t, int, int, const double*, int, const double*, int,
+                               double*, double*, size_t);
+template void col_dots<float>(hipStream_t, int, int, const float*, int, const float*, int, double*,
+                              double*, size_t);
+template void col_resid<double>(hipStream_t, int, int, const double*, int, const double*, int,
+                                const double*, double*, double*, size_t);
+template void col_resid<float>(hipStream_t, int, int, const float*, int, const float*, int,
+                               const double*, double*, double*, size_t);
+
+// ---------------------------------------------------------------------------
+// block helpers
+// ---------------------------------------------------------------------------
+template <typename TS, typename TD>
+__global__ void copy_block_kernel(int n, int b, const TS* __restrict__ S, int lds, TD* __restrict__ D,
+                                  int ldd) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n * b) return;
+  const int i = (int)(idx / b), j = (int)(idx % b);
+  D[(size_t)i * ldd + j] = (TD)S[(size_t)i * lds + j];
+}
+template <typename TS, typename TD>
+void copy_block(hipStream_t st, int n, int b, const TS* S, int lds, TD* D, int ldd) {
+  const size_t tot = (size_t)n * b;
+  if (!tot) return;
+  hipLaunchKernelGGL((copy_block_kernel<TS, TD>), dim3((tot + 255) / 256), dim3(256), 0, st, n, b,
+                     S, lds, D, ldd);
+  SMALGO_HIP(hipGetLastError());
+}
+template void copy_block<double, double>(hipStream_t, int, int, const double*, int, double*, int);
+template void copy_block<float, float>(hipStream_t, int, int, const float*, int, float*, int);
+template void copy_block<double, float>(hipStream_t, int, int, const double*, int, float*, int);
+template void copy_block<float, double>(hipStream_t, int, int, const float*, int, double*, int);
+
+template <typename T>
+__global__ void fill_block_kernel(int n, int b, T* D, int ldd, T v) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n * b) return;
+  D[(size_t)(idx / b) * ldd + idx % b] = v;
+}
+template <typename T>
+void fill_block(hipStream_t st, int n, int b, T* D, int ldd, T v) {
+  const size_t tot = (size_t)n * b;
+  if (!tot) return;
+  hipLaunchKernelGGL((fill_block_kernel<T>), dim3((tot + 255) / 256), dim3(256), 0, st, n, b, D,
+                     ldd, v);
+  SMALGO_HIP(hipGetLastError());
+}
+template void fill_block<double>(hipStream_t, int, int, double*, int, double);
+template void fill_block<float>(hipStream_t, int, int, float*, int, float);
+
+template <typename T>
+__global__ void scale_cols_kernel(int n, int b, T* D, int ldd, const double* s) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n * b) return;
+  const int j = (int)(idx % b);
+  T* p = D + (size_t)(idx / b) * ldd + j;
+  *p = (T)((double)*p * s[j]);
+}
+template <typename T>
+void scale_cols(hipStream_t st, int n, int b, T* D, int ldd, const double* s) {
+  const size_t tot = (size_t)n * b;
+  if (!tot) return;
+  hipLaunchKernelGGL((scale_cols_kernel<T>), dim3((tot + 255) / 256), dim3(256), 0, st, n, b, D,
+                     ldd, s);
+  SMALGO_HIP(hipGetLastError());
+}
+template void scale_cols<double>(hipStream_t, int, int, double*, int, const double*);
+template void scale_cols<float>(hipStream_t, int, int, float*, int, const double*);
+
+__global__ void gather_cols_kernel(int M, int m, const double* __restrict__ S, int lds,
+                                   const int* __restrict__ sel, const int* __restrict__ rowmap,
+                                   double* __restrict__ D, int ldd) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)M * m) return;
+  const int i = (int)(idx / m), j = (int)(idx % m);
+  const int dr = rowmap ? rowmap[i] : i;
+  D[(size_t)dr * ldd + j] = S[(size_t)i + (size_t)sel[j] * lds];
+}
+void gather_cols_colmajor(hipStream_t st, int M, int m, const double* S, int lds, const int* sel,
+                          const int* rowmap, double* D, int ldd) {
+  const size_t tot = (size_t)M * m;
+  if (!tot) return;
+  hipLaunchKernelGGL(gather_cols_kernel, dim3((tot + 255) / 256), dim3(256), 0, st, M, m, S, lds,
+                     sel, rowmap, D, ldd);
+  SMALGO_HIP(hipGetLastError());
+}
+
+__global__ void symmetrize_kernel(int b, double* C, int ldc) {
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b || j >= b || j <= i) return;
+  const double v = 0.5 * (C[(size_t)i * ldc + j] + C[(size_t)j * ldc + i]);
+  C[(size_t)i * ldc + j] = v;
+  C[(size_t)j * ldc + i] = v;
+}
+void symmetrize(hipStream_t st, int b, double* C, int ldc) {
+  if (b <= 1) return;
+  dim3 blk(16, 16), grid((b + 15) / 16, (b + 15) / 16);
+  hipLaunchKernelGGL(symmetrize_kernel, grid, blk, 0, st, b, C, ldc);
+  SMALGO_HIP(hipGetLastError());
+}
+
+__device__ __forceinline__ uint64_t splitmix_at(uint64_t seed, uint64_t i) {
+  uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ void gaussian_kernel(int n, int b, uint64_t seed, T* D, int ldd) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n * b) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);  // column-major counter j*n + i
+  const double u1 = (double)(splitmix_at(seed, idx) >> 11) * (1.0 / 9007199254740992.0);
+  const double u2 =
+      (double)(splitmix_at(seed ^ 0x5DEECE66Dull, idx) >> 11) * (1.0 / 9007199254740992.0);
+  const double g = sqrt(-2.0 * log1p(-u1)) * cos(6.283185307179586 * u2);
+  D[(size_t)i * ldd + j] = (T)g;
+}
+template <typename T>
+void gaussian_block(hipStream_t st, int n, int b, uint64_t seed, T* D, int ldd) {
+  const size_t tot = (size_t)n * b;
+  if (!tot) return;
+  hipLaunchKernelGGL((gaussian_kernel<T>), dim3((tot + 255) / 256), dim3(256), 0, st, n, b, seed, D,
+                     ldd);
+  SMALGO_HIP(hipGetLastError());
+}
+template void gaussian_block<double>(hipStream_t, int, int, uint64_t, double*, int);
+template void gaussian_block<float>(hipStream_t, int, int, uint64_t, float*, int);
+
+// ---------------------------------------------------------------------------
+// layout conversion between the caller's column-major arrays and the
+// device's row-major blocks (32x32 tiles through LDS)
+// ---------------------------------------------------------------------------
+template <typename TS, typename TD>
+__global__ void cm_to_rm_kernel(int n, int b, const TS* __restrict__ S, int lds, TD* __restrict__ D,
+                                int ldd) {
+  __shared__ TD tileb[32][33];
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+  for (int jj = ty; jj < 32; jj += 8) {
+    const int i = i0 + tx, j = j0 + jj;
+    if (i < n && j < b) tileb[jj][tx] = (TD)S[(size_t)j * lds + i];
+  }
+  __syncthreads();
+  for (int ii = ty; ii < 32; ii += 8) {
+    const int i = i0 + ii, j = j0 + tx;
+    if (i < n && j < b) D[(size_t)i * ldd + j] = tileb[tx][ii];
+  }
+}
+template <typename TS, typename TD>
+void cm_to_rm(hipStream_t st, int n, int b, const TS* S, int lds, TD* D, int ldd) {
+  if (!n || !b) return;
+  dim3 grid((n + 31) / 32, (b + 31) / 32), blk(32, 8);
+  hipLaunchKernelGGL((cm_to_rm_kernel<TS, TD>), grid, blk, 0, st, n, b, S, lds, D, ldd);
+  SMALGO_HIP(hipGetLastError());
+}
+template void cm_to_rm<double, double>(hipStream_t, int, int, const double*, int, double*, int);
+template void cm_to_rm<double, float>(hipStream_t, int, int, const double*, int, float*, int);
+
+// D (column-major, ld ldd)[:, j] = S (row-major, ld lds)[:, sel[j]]  (sel may be null)
+template <typename TS>
+__global__ void rm_to_cm_kernel(int n, int k, const TS* __restrict__ S, int lds,
+                                const int* __restrict__ sel, double* __restrict__ D, int ldd) {
+  __shared__ double tileb[32][33];
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  for (int ii = ty; ii < 32; ii += 8) {
+    const int i = i0 + ii, j = j0 + tx;
+    if (i < n && j < k) tileb[ii][tx] = (double)S[(size_t)i * lds + (sel ? sel[j] : j)];
+  }
+  __syncthreads();
+  for (int jj = ty; jj < 32; jj += 8) {
+    const int i = i0 + tx, j = j0 + jj;
+    if (i < n && j < k) D[(size_t)j * ldd + i] = tileb[tx][jj];
+  }
+}
+template <typename TS>
+void rm_to_cm(hipStream_t st, int n, int k, const TS* S, int lds, const int* sel, double* D, int ldd) {
+  if (!n || !k) return;
+  dim3 grid((n + 31) / 32, (k + 31) / 32), blk(32, 8);
+  hipLaunchKernelGGL((rm_to_cm_kernel<TS>), grid, blk, 0, st, n, k, S, lds, sel, D, ldd);
+  SMALGO_HIP(hipGetLastError());
+}
+template void rm_to_cm<double>(hipStream_t, int, int, const double*, int, const int*, double*, int);
+template void rm_to_cm<float>(hipStream_t, int, int, const float*, int, const int*, double*, int);
+
+// D[i, j] = S[i, sel[j]]  (row-major both)
+template <typename T>
+__global__ void gather_cols_rm_kernel(int n, int k, const T* __restrict__ S, int lds,
+                                      const int* __restrict__ sel, T* __restrict__ D, int ldd) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n * k) return;
+  const int i = (int)(idx / k), j = (int)(idx % k);
+  D[(size_t)i * ldd + j] = S[(size_t)i * lds + sel[j]];
+}
+template <typename T>
+void gather_cols_rm(hipStream_t st, int n, int k, const T* S, int lds, const int* sel, T* D, int ldd) {
+  const size_t tot = (size_t)n * k;
+  if (!tot) return;
+  hipLaunchKernelGGL((gather_cols_rm_kernel<T>), dim3((tot + 255) / 256), dim3(256), 0, st, n, k, S,
+                     lds, sel, D, ldd);
+  SMALGO_HIP(hipGetLastError());
+}
+template void gather_cols_rm<double>(hipStream_t, int, int, const double*, int, const int*, double*, int);
+template void gather_cols_rm<float>(hipStream_t, int, int, const float*, int, const int*, float*, int);
+
+__global__ void add_diag_kernel(int b, double* C, int ldc, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < b) C[(size_t)i * ldc + i] += v;
+}
+void add_diag(hipStream_t st, int b, double* C, int ldc, double v) {
+  if (b <= 0) return;
+  hipLaunchKernelGGL(add_diag_kernel, dim3((b + 255) / 256), dim3(256), 0, st, b, C, ldc, v);
+  SMALGO_HIP(hipGetLastError());
+}
+
+// zero the strictly-lower triangle of a column-major b x b matrix (LAPACK
+// routines leave the unused triangle untouched)
+__global__ void zero_lower_cm_kernel(int b, double* C, int ldc) {
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < b && j < b && i > j) C[(size_t)j * ldc + i] = 0.0;
+}
+void zero_lower_cm(hipStream_t st, int b, double* C, int ldc) {
+  if (b <= 1) return;
+  dim3 blk(16, 16), grid((b + 15) / 16, (b + 15) / 16);
+  hipLaunchKernelGGL(zero_lower_cm_kernel, grid, blk, 0, st, b, C, ldc);
+  SMALGO_HIP(hipGetLastError());
+}
+
+int device_cu_count() {
+  static int cached[64] = {0};
+  int dev = 0;
+  SMALGO_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) dev = 0;
+  if (!cached[dev]) {
+    int cu = 0;
+    SMALGO_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+    cached[dev] = cu > 0 ? cu : 256;
+  }
+  return cached[dev];
+}
+
+}  // namespace smalgo

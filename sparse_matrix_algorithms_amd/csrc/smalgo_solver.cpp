@@ -1,0 +1,1166 @@
+// Host drivers for the solve-free interior eigensolvers on one MI355X.
+//
+// polfed           polfed.jl:1128-1158 (driver), :1022-1099 (basis), :892-984 (extraction)
+// chebyshev        chebyshev_subspace_iteration.jl:667-757 (sparse), :771-848 (matvec!)
+//
+// Everything between the caller's arrays and the results stays in HBM: the
+// CSR matrix, the block basis V = [Q_1 … Q_m] (row-major n x (mmax·s), block j
+// at columns [j·s, j·s + r_j)), the Clenshaw ping-pong buffers, the lifted Ritz
+// vectors.  Small dense FP64 steps (Cholesky, triangular inverse, symmetric
+// eigensolve) run through rocSOLVER on the same stream; the block-tridiagonal
+// T and the rank decisions are assembled on the host from r x r blocks.
+#include "smalgo_solver.h"
+
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <charconv>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <limits>
+#include <numeric>
+
+#include "smalgo_coeffs.h"
+
+namespace smalgo {
+
+static inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// ---------------------------------------------------------------------------
+// Dense (rocSOLVER, column-major FP64)
+// ---------------------------------------------------------------------------
+static void check_rb(rocblas_status s, const char* what) {
+  if (s != rocblas_status_success)
+    throw Error(-4, std::string(what) + " failed with rocblas status " + std::to_string((int)s));
+}
+
+int Dense::potrf_upper(int b, double* A, int lda) {
+  check_rb(rocsolver_dpotrf(cx.blas, rocblas_fill_upper, b, A, lda, info.p), "rocsolver_dpotrf");
+  int h = 0;
+  SMALGO_HIP(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, cx.stream));
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  return h;
+}
+
+void Dense::trtri_upper(int b, double* A, int lda) {
+  check_rb(rocsolver_dtrtri(cx.blas, rocblas_fill_upper, rocblas_diagonal_non_unit, b, A, lda,
+                            info.p),
+           "rocsolver_dtrtri");
+}
+
+void Dense::syevd(int n, double* A, int lda, double* w, double* work_e, bool upper) {
+  check_rb(rocsolver_dsyevd(cx.blas, rocblas_evect_original,
+                            upper ? rocblas_fill_upper : rocblas_fill_lower, n, A, lda, w, work_e,
+                            info.p),
+           "rocsolver_dsyevd");
+  int h = 0;
+  SMALGO_HIP(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, cx.stream));
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  if (h != 0) throw Error(-4, "rocsolver_dsyevd did not converge (info=" + std::to_string(h) + ")");
+}
+
+// ---------------------------------------------------------------------------
+// phase timer
+// ---------------------------------------------------------------------------
+size_t PhaseTimer::begin(const std::string& name) {
+  hipEvent_t a, b;
+  SMALGO_HIP(hipEventCreate(&a));
+  SMALGO_HIP(hipEventCreate(&b));
+  SMALGO_HIP(hipEventRecord(a, st));
+  ev.push_back({name, {a, b}});
+  return ev.size() - 1;
+}
+void PhaseTimer::end(size_t h) { SMALGO_HIP(hipEventRecord(ev[h].second.second, st)); }
+std::map<std::string, double> PhaseTimer::resolve() {
+  std::map<std::string, double> out;
+  SMALGO_HIP(hipStreamSynchronize(st));
+  for (auto& e : ev) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e.second.first, e.second.second) == hipSuccess)
+      out[e.first] += ms * 1e-3;
+  }
+  return out;
+}
+PhaseTimer::~PhaseTimer() {
+  for (auto& e : ev) {
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// filter tile choice: widest aligned vector, then enough lanes to cover b
+// ---------------------------------------------------------------------------
+FilterTile choose_tile(size_t elem, int b, std::initializer_list<const void*> ptrs,
+                       std::initializer_list<int> lds) {
+  int vec = elem == 4 ? 4 : 2;
+  auto ok = [&](int v) {
+    for (const void* p : ptrs)
+      if (p && reinterpret_cast<uintptr_t>(p) % (elem * v)) return false;
+    for (int ld : lds)
+      if (ld % v) return false;
+    return true;
+  };
+  while (vec > 1 && (!ok(vec) || b < vec)) vec /= 2;
+  int need = (b + vec - 1) / vec;
+  int lpr = 2;
+  while (lpr < need && lpr < 64) lpr *= 2;
+  return FilterTile{vec, lpr};
+}
+
+// ---------------------------------------------------------------------------
+// operator application
+// ---------------------------------------------------------------------------
+static void callback_apply(Ctx& cx, const Op& op, int b, const double* X, int ldx, double* Y,
+                           int ldy) {
+  // device row-major -> host column-major -> user -> back
+  const int n = op.n;
+  std::vector<double> hx((size_t)n * b), hy((size_t)n * b, 0.0);
+  DBuf<double> tmp((size_t)n * b);
+  rm_to_cm<double>(cx.stream, n, b, X, ldx, nullptr, tmp.p, n);
+  SMALGO_HIP(hipMemcpyAsync(hx.data(), tmp.p, sizeof(double) * hx.size(), hipMemcpyDeviceToHost,
+                            cx.stream));
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  const int rc = op.fn(op.user, n, b, hx.data(), n, hy.data(), n);
+  if (rc != 0) throw Error(-5, "matblock callback returned " + std::to_string(rc));
+  SMALGO_HIP(hipMemcpyAsync(tmp.p, hy.data(), sizeof(double) * hy.size(), hipMemcpyHostToDevice,
+                            cx.stream));
+  cm_to_rm<double, double>(cx.stream, n, b, tmp.p, n, Y, ldy);
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+}
+
+template <typename T>
+void apply_A(Ctx& cx, const Op& op, int b, const T* X, int ldx, T* Y, int ldy) {
+  if (op.is_callback()) {
+    if constexpr (std::is_same<T, double>::value) {
+      callback_apply(cx, op, b, X, ldx, Y, ldy);
+      return;
+    } else {
+      throw Error(-3, "matrix-free operators run in FP64 only");
+    }
+  }
+  const FilterTile t = choose_tile(sizeof(T), b, {X, Y}, {ldx, ldy});
+  launch_clenshaw_step<T>(cx.stream, *op.A, b, nullptr, 0, T(0), X, ldx, T(1), T(0), nullptr, 0,
+                          T(0), Y, ldy, t);
+}
+template void apply_A<double>(Ctx&, const Op&, int, const double*, int, double*, int);
+template void apply_A<float>(Ctx&, const Op&, int, const float*, int, float*, int);
+
+// One Clenshaw step out = cq*Q + a1*(A X1) + b1*X1 - s2*X2 with either
+// operator kind.  X1/X2 are given as (pointer, implicit multiple of Q).
+template <typename T>
+struct FBlk {
+  const T* p;
+  double qs;
+};
+
+template <typename T>
+static void filter_step(Ctx& cx, const Op& op, int b, const T* Q, int ldq, double cq, double a1,
+                        double b1, FBlk<T> X1, FBlk<T> X2, int ldb, T* out, int ldo, double* pbuf,
+                        const FilterTile* tile) {
+  const T* x1p = X1.p;
+  int ld1 = ldb;
+  double a1e = a1, b1e = b1;
+  if (!x1p) {
+    if (X1.qs == 0.0) {
+      a1e = b1e = 0.0;
+    } else {
+      x1p = Q;
+      ld1 = ldq;
+      a1e = a1 * X1.qs;
+      b1e = b1 * X1.qs;
+    }
+  }
+  const T* x2p = X2.p;
+  double cqe = cq, s2e = 1.0;
+  if (!x2p) {
+    cqe = cq - X2.qs;
+    s2e = 0.0;
+  }
+  if (op.is_callback()) {
+    if constexpr (std::is_same<T, double>::value) {
+      const double* P = nullptr;
+      if (x1p) {
+        callback_apply(cx, op, b, x1p, ld1, pbuf, ldb);
+        P = pbuf;
+      } else {
+        a1e = 0.0;
+      }
+      launch_clenshaw_step_pre(cx.stream, op.n, b, Q, ldq, cqe, P ? P : nullptr, ldb, x1p, ld1,
+                               P ? a1e : 0.0, b1e, x2p, ldb, s2e, out, ldo);
+      return;
+    } else {
+      throw Error(-3, "matrix-free operators run in FP64 only");
+    }
+  }
+  FilterTile t = tile ? *tile : choose_tile(sizeof(T), b, {Q, x1p, x2p, out}, {ldq, ld1, ldb, ldo});
+  launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, (T)cqe, x1p, ld1, (T)a1e, (T)b1e, x2p, ldb,
+                          (T)s2e, out, ldo, t);
+}
+
+template <typename T>
+void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::vector<double>& co,
+                  double smin, double smax, T* Y, int ldy, T* buf0, T* buf1, int ldb,
+                  double* pbuf) {
+  // Block Clenshaw exactly as apply_poly_block! (polfed.jl:464-501):
+  //   B_{K+1} = B_{K+2} = 0
+  //   for k = K+1 .. 2:  TMP = c[k-1] Q + α A B1 + β B1 - B2;  (B1,B2) <- (TMP,B1)
+  //   Y = c0 Q + invd A B1 - c invd B1 - B2
+  // with invd = 2/(smax-smin), α = 2 invd, β = -2 c invd.  The first two
+  // steps have B1 = c_K Q and B2 ∈ {0, c_K Q}; they are folded into the
+  // Q term instead of materialised, so the zero SpMM of the reference's first
+  // step is never launched.  From the third step on TMP overwrites B2 in place.
+  const int K = (int)co.size() - 1;
+  const double c = 0.5 * (smin + smax);
+  const double invd = 2.0 / (smax - smin);
+  const double al = 2.0 * invd, be = -2.0 * c * invd;
+  FBlk<T> B1{nullptr, 0.0}, B2{nullptr, 0.0};
+  T* bufs[2] = {buf0, buf1};
+  FilterTile tile{};
+  const FilterTile* tp = nullptr;
+  if (!op.is_callback()) {
+    tile = choose_tile(sizeof(T), b, {Q, buf0, buf1, Y}, {ldq, ldb, ldy});
+    auto it = cx.tuned.find(((long long)sizeof(T) << 40) ^ ((long long)b << 20) ^ op.mat_id);
+    if (it != cx.tuned.end()) tile = it->second;
+    tp = &tile;
+  }
+  for (int k = K + 1; k >= 2; --k) {
+    const double ck = co[k - 1];
+    if (k == K + 1) {
+      B1 = {nullptr, ck};
+      B2 = {nullptr, 0.0};
+      continue;
+    }
+    T* out = B2.p ? const_cast<T*>(B2.p) : (B1.p == bufs[0] ? bufs[1] : bufs[0]);
+    filter_step<T>(cx, op, b, Q, ldq, ck, al, be, B1, B2, ldb, out, ldb, pbuf, tp);
+    B2 = B1;
+    B1 = {out, 0.0};
+  }
+  filter_step<T>(cx, op, b, Q, ldq, co[0], invd, -c * invd, B1, B2, ldb, Y, ldy, pbuf, tp);
+}
+template void apply_filter<double>(Ctx&, const Op&, int, const double*, int,
+                                   const std::vector<double>&, double, double, double*, int,
+                                   double*, double*, int, double*);
+template void apply_filter<float>(Ctx&, const Op&, int, const float*, int,
+                                  const std::vector<double>&, double, double, float*, int, float*,
+                                  float*, int, double*);
+
+// ---------------------------------------------------------------------------
+// CholQR2 with a shifted-CholQR3 fallback (Fukaya et al., SIAM J. Sci.
+// Comput. 42 (2020)): R from the FP64 Cholesky of the Gram matrix, Q = W R^{-1}.
+// Replaces Householder qr! in orth_block_polfed!/block_qr_factor
+// (polfed.jl:561-594, :644-685) and thin_orth! (chebyshev_subspace_iteration.jl:223-231).
+// Returns the count rank r = #{|R_ii| > rtol·max|R_ii|}; Q gets all b columns,
+// the caller keeps the first r (the reference's rule, polfed.jl:679-684).
+// ---------------------------------------------------------------------------
+struct OrthWork {
+  DBuf<double> G, Rinv, gemm_work;
+  std::vector<double> hG;
+};
+
+template <typename T>
+static int cholqr(Ctx& cx, Dense& dn, OrthWork& ow, int n, int b, const T* W, int ldw, T* Qout,
+                  int ldq, T* tmp, int ldt, double rtol, std::vector<double>* Rout) {
+  const double u = std::is_same<T, float>::value ? 0x1p-24 : 0x1p-53;
+  ow.G.ensure((size_t)b * b);
+  ow.Rinv.ensure((size_t)b * b);
+  ow.gemm_work.ensure(gemm_tn_work_elems(n, b, b));
+  ow.hG.resize((size_t)b * b);
+  std::vector<double> Racc((size_t)b * b, 0.0), Rp((size_t)b * b);
+  for (int i = 0; i < b; ++i) Racc[(size_t)i * b + i] = 1.0;
+  const T* cur = W;
+  int ldc = ldw;
+  bool shifted = false;
+  int passes = 2;
+  for (int pass = 0; pass < passes; ++pass) {
+    // never read and write the same block in one GEMM: alternate tmp <-> Qout
+    T* dst = (cur == tmp) ? Qout : tmp;
+    int ldd = (cur == tmp) ? ldq : ldt;
+    gemm_tn<T, T, double>(cx.stream, n, b, b, 1.0, cur, ldc, cur, ldc, 0.0, ow.G.p, b,
+                          ow.gemm_work.p, ow.gemm_work.n);
+    SMALGO_HIP(hipMemcpyAsync(ow.hG.data(), ow.G.p, sizeof(double) * b * b, hipMemcpyDeviceToHost,
+                              cx.stream));
+    int info = dn.potrf_upper(b, ow.G.p, b);
+    double tr = 0.0;
+    for (int i = 0; i < b; ++i) tr += std::max(0.0, ow.hG[(size_t)i * b + i]);
+    double shift = 11.0 * ((double)n * b + (double)b * (b + 1)) * u * tr;
+    int tries = 0;
+    while (info != 0) {
+      if (tr == 0.0 || !std::isfinite(tr) || ++tries > 6) {
+        if (Rout) Rout->assign((size_t)b * b, 0.0);
+        return 0;
+      }
+      shifted = true;
+      SMALGO_HIP(hipMemcpyAsync(ow.G.p, ow.hG.data(), sizeof(double) * b * b,
+                                hipMemcpyHostToDevice, cx.stream));
+      add_diag(cx.stream, b, ow.G.p, b, shift);
+      info = dn.potrf_upper(b, ow.G.p, b);
+      shift *= 100.0;
+    }
+    if (shifted && passes == 2) passes = 3;
+    zero_lower_cm(cx.stream, b, ow.G.p, b);
+    SMALGO_HIP(hipMemcpyAsync(Rp.data(), ow.G.p, sizeof(double) * b * b, hipMemcpyDeviceToHost,
+                              cx.stream));
+    dn.trtri_upper(b, ow.G.p, b);
+    // dst = cur · R^{-1}; R^{-1} column-major -> transB
+    gemm_nn<T, double, T>(cx.stream, n, b, b, 1.0, cur, ldc, ow.G.p, b, true, 0.0, dst, ldd);
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+    // Racc <- Rp · Racc   (both upper triangular, column-major)
+    std::vector<double> Rn((size_t)b * b, 0.0);
+    for (int j = 0; j < b; ++j)
+      for (int i = 0; i <= j; ++i) {
+        double s = 0.0;
+        for (int k = i; k <= j; ++k) s += Rp[(size_t)k * b + i] * Racc[(size_t)j * b + k];
+        Rn[(size_t)j * b + i] = s;
+      }
+    Racc.swap(Rn);
+    cur = dst;
+    ldc = ldd;
+  }
+  if (cur != Qout) copy_block<T, T>(cx.stream, n, b, cur, ldc, Qout, ldq);
+  double r0 = 0.0;
+  for (int i = 0; i < b; ++i) r0 = std::max(r0, std::fabs(Racc[(size_t)i * b + i]));
+  int r = 0;
+  if (r0 > 0)
+    for (int i = 0; i < b; ++i) r += std::fabs(Racc[(size_t)i * b + i]) > rtol * r0 ? 1 : 0;
+  if (Rout) *Rout = std::move(Racc);
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// formatting of the progress line (polfed.jl:991-1000)
+// ---------------------------------------------------------------------------
+std::string format_julia_float(double v) {
+  if (std::isinf(v)) return v > 0 ? "Inf" : "-Inf";
+  if (std::isnan(v)) return "NaN";
+  char buf[64];
+  auto r = std::to_chars(buf, buf + sizeof(buf), v);
+  std::string s(buf, r.ptr);
+  if (s.find_first_of(".eEn") == std::string::npos) s += ".0";
+  return s;
+}
+
+std::string progress_line(int m, const std::vector<double>& lam, const std::vector<double>& res) {
+  if (res.empty()) return "m=" + std::to_string(m) + "  inside=0  maxres=Inf";
+  std::vector<double> rs(res);
+  std::sort(rs.begin(), rs.end());
+  const int L = (int)rs.size();
+  auto ridx = [&](double f) {
+    int k = (int)std::nearbyint(f * L);
+    return std::min(std::max(k, 1), L) - 1;
+  };
+  return "m=" + std::to_string(m) + "  inside=" + std::to_string(lam.size()) +
+         "  r90=" + format_julia_float(rs[ridx(0.9)]) + "  r95=" + format_julia_float(rs[ridx(0.95)]) +
+         "  maxres=" + format_julia_float(rs.back());
+}
+
+static std::vector<int> stable_argsort(const std::vector<double>& v, bool descending = false) {
+  std::vector<int> p(v.size());
+  std::iota(p.begin(), p.end(), 0);
+  if (descending)
+    std::stable_sort(p.begin(), p.end(), [&](int a, int b) { return v[a] > v[b]; });
+  else
+    std::stable_sort(p.begin(), p.end(), [&](int a, int b) { return v[a] < v[b]; });
+  return p;
+}
+
+static DBuf<int> upload_idx(Ctx& cx, const std::vector<int>& v) {
+  DBuf<int> d(std::max<size_t>(v.size(), 1));
+  if (!v.empty())
+    SMALGO_HIP(hipMemcpyAsync(d.p, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice,
+                              cx.stream));
+  return d;
+}
+
+// Rayleigh quotients and residuals of the columns of X (FP64):
+// λ_j = x_jᵀ A x_j, res_j = ‖A x_j − λ_j x_j‖ (rayleigh!, polfed.jl:791-830)
+struct RayleighWork {
+  DBuf<double> AX, lam, res, red;
+};
+static void rayleigh(Ctx& cx, const Op& op, int n, int k, const double* X, int ldx, RayleighWork& w,
+                     std::vector<double>& lam, std::vector<double>& res) {
+  const int ld = round_up(std::max(k, 1), 4);
+  w.AX.ensure((size_t)n * ld);
+  w.lam.ensure(std::max(k, 1));
+  w.res.ensure(std::max(k, 1));
+  w.red.ensure(colred_work_elems(n, k));
+  apply_A<double>(cx, op, k, X, ldx, w.AX.p, ld);
+  col_dots<double>(cx.stream, n, k, X, ldx, w.AX.p, ld, w.lam.p, w.red.p, w.red.n);
+  col_resid<double>(cx.stream, n, k, X, ldx, w.AX.p, ld, w.lam.p, w.res.p, w.red.p, w.red.n);
+  lam.resize(k);
+  res.resize(k);
+  if (k) {
+    SMALGO_HIP(hipMemcpyAsync(lam.data(), w.lam.p, sizeof(double) * k, hipMemcpyDeviceToHost,
+                              cx.stream));
+    SMALGO_HIP(hipMemcpyAsync(res.data(), w.res.p, sizeof(double) * k, hipMemcpyDeviceToHost,
+                              cx.stream));
+  }
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+}
+
+// Original-space Rayleigh–Ritz (refine_original_rr, polfed.jl:837-880) on the
+// k columns of X (row-major FP64).  Output Y (n x kq, ld ldy, sorted by λ).
+struct RRWork {
+  OrthWork ow;
+  DBuf<double> Q, tmp, AQ, H, w, e, gw;
+  RayleighWork rw;
+};
+static int refine_rr(Ctx& cx, Dense& dn, const Op& op, int n, int k, const double* X, int ldx,
+                     RRWork& w, DBuf<double>& Y, int& ldy, std::vector<double>& lam,
+                     std::vector<double>& res, double rtol = 1e-12) {
+  const int ld = round_up(std::max(k, 1), 4);
+  w.Q.ensure((size_t)n * ld);
+  w.tmp.ensure((size_t)n * ld);
+  int kq = cholqr<double>(cx, dn, w.ow, n, k, X, ldx, w.Q.p, ld, w.tmp.p, ld, rtol, nullptr);
+  if (kq == 0) {
+    lam.clear();
+    res.clear();
+    return 0;
+  }
+  w.AQ.ensure((size_t)n * ld);
+  apply_A<double>(cx, op, kq, w.Q.p, ld, w.AQ.p, ld);
+  w.H.ensure((size_t)kq * kq);
+  w.gw.ensure(gemm_tn_work_elems(n, kq, kq));
+  gemm_tn<double, double, double>(cx.stream, n, kq, kq, 1.0, w.Q.p, ld, w.AQ.p, ld, 0.0, w.H.p, kq,
+                                  w.gw.p, w.gw.n);
+  symmetrize(cx.stream, kq, w.H.p, kq);
+  w.w.ensure(kq);
+  w.e.ensure(kq);
+  dn.syevd(kq, w.H.p, kq, w.w.p, w.e.p, true);
+  ldy = ld;
+  Y.ensure((size_t)n * ld);
+  gemm_nn<double, double, double>(cx.stream, n, kq, kq, 1.0, w.Q.p, ld, w.H.p, kq, true, 0.0, Y.p,
+                                  ld);
+  std::vector<double> l, r;
+  rayleigh(cx, op, n, kq, Y.p, ld, w.rw, l, r);
+  std::vector<int> p = stable_argsort(l);
+  lam.resize(kq);
+  res.resize(kq);
+  for (int i = 0; i < kq; ++i) {
+    lam[i] = l[p[i]];
+    res[i] = r[p[i]];
+  }
+  DBuf<int> dp = upload_idx(cx, p);
+  gather_cols_rm<double>(cx.stream, n, kq, Y.p, ld, dp.p, w.tmp.p, ld);
+  SMALGO_HIP(hipMemcpyAsync(Y.p, w.tmp.p, sizeof(double) * (size_t)n * ld, hipMemcpyDeviceToDevice,
+                            cx.stream));
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  return kq;
+}
+
+// ---------------------------------------------------------------------------
+// one-call block kernels on host column-major arrays
+// ---------------------------------------------------------------------------
+template <typename T>
+static void upload_cm(Ctx& cx, int n, int b, const double* H, int ldh, T* D, int ldd) {
+  DBuf<double> cm((size_t)n * std::max(b, 1));
+  for (int j = 0; j < b; ++j)
+    SMALGO_HIP(hipMemcpyAsync(cm.p + (size_t)j * n, H + (size_t)j * ldh, sizeof(double) * n,
+                              hipMemcpyHostToDevice, cx.stream));
+  cm_to_rm<double, T>(cx.stream, n, b, cm.p, n, D, ldd);
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+}
+template <typename T>
+static void download_cm(Ctx& cx, int n, int b, const T* D, int ldd, double* H, int ldh) {
+  DBuf<double> cm((size_t)n * std::max(b, 1));
+  rm_to_cm<T>(cx.stream, n, b, D, ldd, nullptr, cm.p, n);
+  for (int j = 0; j < b; ++j)
+    SMALGO_HIP(hipMemcpyAsync(H + (size_t)j * ldh, cm.p + (size_t)j * n, sizeof(double) * n,
+                              hipMemcpyDeviceToHost, cx.stream));
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+}
+
+template <typename T>
+static void orth_against_t(Ctx& cx, int n, int b, double* W, int ldw, int m, const double* Q,
+                           int ldq) {
+  const int lb = round_up(std::max(b, 1), 4), lm = round_up(std::max(m, 1), 4);
+  DBuf<T> dW((size_t)n * lb), dQ((size_t)n * lm), C((size_t)lm * lb);
+  DBuf<double> gw(gemm_tn_work_elems(n, m, b));
+  upload_cm<T>(cx, n, b, W, ldw, dW.p, lb);
+  upload_cm<T>(cx, n, m, Q, ldq, dQ.p, lm);
+  gemm_tn<T, T, T>(cx.stream, n, m, b, 1.0, dQ.p, lm, dW.p, lb, 0.0, C.p, lb, gw.p, gw.n);
+  gemm_nn<T, T, T>(cx.stream, n, m, b, -1.0, dQ.p, lm, C.p, lb, false, 1.0, dW.p, lb);
+  download_cm<T>(cx, n, b, dW.p, lb, W, ldw);
+}
+void orth_against_host(Ctx& cx, int n, int b, double* W, int ldw, int m, const double* Q, int ldq,
+                       int precision) {
+  if (m == 0 || b == 0) return;
+  if (precision == SMALGO_PREC_FP32_REFINE64)
+    orth_against_t<float>(cx, n, b, W, ldw, m, Q, ldq);
+  else
+    orth_against_t<double>(cx, n, b, W, ldw, m, Q, ldq);
+}
+
+template <typename T>
+static void block_qr_t(Ctx& cx, int n, int b, const double* W, int ldw, double rtol, double* Q,
+                       int ldq, double* R, int ldr, int* rank) {
+  const int lb = round_up(std::max(b, 1), 4);
+  DBuf<T> dW((size_t)n * lb), dQ((size_t)n * lb), tmp((size_t)n * lb);
+  upload_cm<T>(cx, n, b, W, ldw, dW.p, lb);
+  Dense dn(cx);
+  OrthWork ow;
+  std::vector<double> Rh;
+  *rank = cholqr<T>(cx, dn, ow, n, b, dW.p, lb, dQ.p, lb, tmp.p, lb, rtol, &Rh);
+  download_cm<T>(cx, n, b, dQ.p, lb, Q, ldq);
+  if (R)
+    for (int j = 0; j < b; ++j)
+      for (int i = 0; i < b; ++i) R[(size_t)j * ldr + i] = Rh[(size_t)j * b + i];
+}
+void block_qr_host(Ctx& cx, int n, int b, const double* W, int ldw, double rtol, int precision,
+                   double* Q, int ldq, double* R, int ldr, int* rank) {
+  if (precision == SMALGO_PREC_FP32_REFINE64)
+    block_qr_t<float>(cx, n, b, W, ldw, rtol, Q, ldq, R, ldr, rank);
+  else
+    block_qr_t<double>(cx, n, b, W, ldw, rtol, Q, ldq, R, ldr, rank);
+}
+
+void build_X_host(Ctx& cx, int n, int M, int k, const double* V, int ldv, const double* U, int ldu,
+                  double* X, int ldx) {
+  const int lm = round_up(std::max(M, 1), 4), lk = round_up(std::max(k, 1), 4);
+  DBuf<double> dV((size_t)n * lm), dU((size_t)M * lk), dX((size_t)n * lk);
+  upload_cm<double>(cx, n, M, V, ldv, dV.p, lm);
+  upload_cm<double>(cx, M, k, U, ldu, dU.p, lk);
+  gemm_nn<double, double, double>(cx.stream, n, M, k, 1.0, dV.p, lm, dU.p, lk, false, 0.0, dX.p, lk);
+  download_cm<double>(cx, n, k, dX.p, lk, X, ldx);
+}
+
+void rayleigh_host(Ctx& cx, const Op& op, int k, const double* X, int ldx, double* lam, double* res) {
+  const int n = op.n, lk = round_up(std::max(k, 1), 4);
+  DBuf<double> dX((size_t)n * lk);
+  upload_cm<double>(cx, n, k, X, ldx, dX.p, lk);
+  RayleighWork w;
+  std::vector<double> l, r;
+  rayleigh(cx, op, n, k, dX.p, lk, w, l, r);
+  std::copy(l.begin(), l.end(), lam);
+  std::copy(r.begin(), r.end(), res);
+}
+
+void residuals_host(Ctx& cx, const Op& op, int k, const double* X, int ldx, const double* lam,
+                    double* res) {
+  const int n = op.n, lk = round_up(std::max(k, 1), 4);
+  DBuf<double> dX((size_t)n * lk), AX((size_t)n * lk), dl(std::max(k, 1)), dr(std::max(k, 1)),
+      red(colred_work_elems(n, k));
+  upload_cm<double>(cx, n, k, X, ldx, dX.p, lk);
+  SMALGO_HIP(hipMemcpyAsync(dl.p, lam, sizeof(double) * k, hipMemcpyHostToDevice, cx.stream));
+  apply_A<double>(cx, op, k, dX.p, lk, AX.p, lk);
+  col_resid<double>(cx.stream, n, k, dX.p, lk, AX.p, lk, dl.p, dr.p, red.p, red.n);
+  SMALGO_HIP(hipMemcpyAsync(res, dr.p, sizeof(double) * k, hipMemcpyDeviceToHost, cx.stream));
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+}
+
+Result* refine_rr_host(Ctx& cx, const Op& op, int k, const double* X, int ldx) {
+  const int n = op.n, lk = round_up(std::max(k, 1), 4);
+  DBuf<double> dX((size_t)n * lk);
+  upload_cm<double>(cx, n, k, X, ldx, dX.p, lk);
+  Dense dn(cx);
+  RRWork w;
+  DBuf<double> Y;
+  int ldy = 4;
+  Result* out = new Result();
+  out->n = n;
+  out->device = cx.device;
+  const int kq = refine_rr(cx, dn, op, n, k, dX.p, lk, w, Y, ldy, out->lam, out->res);
+  if (kq > 0) {
+    SMALGO_HIP(hipMalloc((void**)&out->Xdev, sizeof(double) * (size_t)n * kq));
+    rm_to_cm<double>(cx.stream, n, kq, Y.p, ldy, nullptr, out->Xdev, n);
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// POLFED
+// ---------------------------------------------------------------------------
+template <typename T>
+static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smalgo_polfed_params& p,
+                          const double* Q1host) {
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  const int n = op.n;
+  if (nev < 1) throw Error(-1, "nev must be positive");
+  if (degree < 0) throw Error(-1, "degree must be nonnegative");
+  if (p.s < 1) throw Error(-1, "block size s must be positive");
+  if (p.s > n) throw Error(-1, "block size s must not exceed n");
+  double smin = p.smin, smax = p.smax;
+  if (!p.has_bounds) {
+    if (op.is_callback()) throw Error(-1, "Matrix-free POLFED requires spectral bounds smin and smax");
+    smin = op.A->gersh_min;
+    smax = op.A->gersh_max;
+  }
+  if (!(smax > smin)) throw Error(-1, "need smin < smax");
+  const std::vector<double> coeffs =
+      build_coeffs(degree, smin, smax, p.sigma, p.emin, p.emax, p.window_type, p.jackson != 0);
+  const int s = p.s;
+  const int mmax = p.mmax > 0 ? p.mmax : (int)std::ceil(2.8 * nev / (double)s);
+  const int lds = round_up(s, 4);
+  const int ldV = round_up(mmax * s, 4);
+  PhaseTimer tm(cx.stream);
+  Dense dn(cx);
+  OrthWork ow;
+
+  DBuf<T> V((size_t)n * ldV), W((size_t)n * lds), b0((size_t)n * lds), b1((size_t)n * lds);
+  DBuf<double> pbuf;
+  if (op.is_callback()) pbuf.alloc((size_t)n * lds);
+  fill_block<T>(cx.stream, n, ldV, V.p, ldV, T(0));
+  // ---- start block: orth_block_polfed!(randn(n,s); rtol) (polfed.jl:1133-1134)
+  int r_cur = 0;
+  {
+    DBuf<double> q64((size_t)n * lds), t64((size_t)n * lds), o64((size_t)n * lds);
+    if (Q1host) {
+      DBuf<double> cm((size_t)n * s);
+      SMALGO_HIP(hipMemcpyAsync(cm.p, Q1host, sizeof(double) * (size_t)n * s,
+                                hipMemcpyHostToDevice, cx.stream));
+      cm_to_rm<double, double>(cx.stream, n, s, cm.p, n, q64.p, lds);
+      SMALGO_HIP(hipStreamSynchronize(cx.stream));
+    } else {
+      gaussian_block<double>(cx.stream, n, s, p.seed, q64.p, lds);
+    }
+    r_cur = cholqr<double>(cx, dn, ow, n, s, q64.p, lds, o64.p, lds, t64.p, lds, p.rtol, nullptr);
+    copy_block<double, T>(cx.stream, n, r_cur, o64.p, lds, V.p, ldV);
+  }
+  std::vector<int> ranks;
+  std::vector<std::vector<double>> Ablk, Bblk;  // column-major r_j x r_j, r_{j+1} x r_j
+  DBuf<double> Adev((size_t)s * s), Bprev((size_t)s * s);
+  DBuf<T> Cbuf((size_t)ldV * lds);
+  size_t gwn = 0;
+  for (int k = 1; k <= mmax; ++k) gwn = std::max(gwn, gemm_tn_work_elems(n, k * s, s));
+  DBuf<double> gw(gwn);
+  std::vector<double> hA((size_t)s * s), R;
+  int r_prev = 0;
+  double t_filter = 0;
+  long long filter_steps = 0;
+  double gemm_flops = 0;
+  for (int j = 0; j < mmax && r_cur > 0; ++j) {
+    const int r = r_cur;
+    ranks.push_back(r);
+    T* Qj = V.p + (size_t)j * s;
+    size_t hf = tm.begin("filter");
+    apply_filter<T>(cx, op, r, Qj, ldV, coeffs, smin, smax, W.p, lds, b0.p, b1.p, lds, pbuf.p);
+    tm.end(hf);
+    filter_steps += degree;  // K fused steps (the all-zero first step is skipped)
+    size_t ho = tm.begin("orth");
+    // W -= Q_{j-1} B_{j-1}ᵀ   (polfed.jl:1073-1075)
+    if (j > 0 && r_prev > 0) {
+      const T* Qp = V.p + (size_t)(j - 1) * s;
+      gemm_nn<T, double, T>(cx.stream, n, r_prev, r, -1.0, Qp, ldV, Bprev.p, s, false, 1.0, W.p,
+                            lds);
+      gemm_flops += 2.0 * n * r_prev * r;
+    }
+    // A_j = sym(Q_jᵀ W)   (polfed.jl:1078)
+    gemm_tn<T, T, double>(cx.stream, n, r, r, 1.0, Qj, ldV, W.p, lds, 0.0, Adev.p, r, gw.p, gw.n);
+    symmetrize(cx.stream, r, Adev.p, r);
+    gemm_flops += 2.0 * n * r * r;
+    std::vector<double> Aj((size_t)r * r);
+    SMALGO_HIP(hipMemcpyAsync(Aj.data(), Adev.p, sizeof(double) * r * r, hipMemcpyDeviceToHost,
+                              cx.stream));
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+    Ablk.push_back(std::move(Aj));
+    if (j == mmax - 1) {  // Q_{mmax+1} and B_mmax never enter T or the Ritz vectors
+      tm.end(ho);
+      break;
+    }
+    // W -= Q_j A_j   (polfed.jl:1082)
+    gemm_nn<T, double, T>(cx.stream, n, r, r, -1.0, Qj, ldV, Adev.p, r, false, 1.0, W.p, lds);
+    gemm_flops += 2.0 * n * r * r;
+    // reorth_full!(W, Qhist, Q): against every stored block, then once more
+    // against Q_j (polfed.jl:623-631; Qhist already holds Q_j, 1067)
+    const int mall = (j + 1) * s;
+    for (int pass = 0; pass < p.reorth_passes; ++pass) {
+      gemm_tn<T, T, T>(cx.stream, n, mall, r, 1.0, V.p, ldV, W.p, lds, 0.0, Cbuf.p, lds, gw.p,
+                       gw.n);
+      gemm_nn<T, T, T>(cx.stream, n, mall, r, -1.0, V.p, ldV, Cbuf.p, lds, false, 1.0, W.p, lds);
+      gemm_tn<T, T, T>(cx.stream, n, r, r, 1.0, Qj, ldV, W.p, lds, 0.0, Cbuf.p, lds, gw.p, gw.n);
+      gemm_nn<T, T, T>(cx.stream, n, r, r, -1.0, Qj, ldV, Cbuf.p, lds, false, 1.0, W.p, lds);
+      gemm_flops += 4.0 * n * mall * r + 4.0 * n * r * r;
+    }
+    // [Q_{j+1}, B_j] = block_qr_factor(W; rtol)   (polfed.jl:1090)
+    T* Qn = V.p + (size_t)(j + 1) * s;
+    const int rn = cholqr<T>(cx, dn, ow, n, r, W.p, lds, Qn, ldV, b0.p, lds, p.rtol, &R);
+    gemm_flops += 2.0 * 2.0 * n * r * r * 2;
+    tm.end(ho);
+    if (rn == 0) break;  // rank collapse (polfed.jl:1091)
+    if (rn < r) fill_block<T>(cx.stream, n, s - rn, Qn + rn, ldV, T(0));
+    // B_j = R[1:rn, 1:r]  (column-major, stored with ld s on the device)
+    std::vector<double> Bj((size_t)rn * r);
+    for (int c = 0; c < r; ++c)
+      for (int i = 0; i < rn; ++i) Bj[(size_t)c * rn + i] = R[(size_t)c * r + i];
+    // device copy as row-major-readable Bᵀ: element (k, i) at k*s + i = B(i, k)
+    std::vector<double> Bt((size_t)s * s, 0.0);
+    for (int i = 0; i < rn; ++i)
+      for (int k = 0; k < r; ++k) Bt[(size_t)k * s + i] = Bj[(size_t)k * rn + i];
+    SMALGO_HIP(hipMemcpyAsync(Bprev.p, Bt.data(), sizeof(double) * s * s, hipMemcpyHostToDevice,
+                              cx.stream));
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+    Bblk.push_back(std::move(Bj));
+    r_prev = r;
+    r_cur = rn;
+  }
+  const int nb = (int)ranks.size();
+  // ---------------- extraction (polfed.jl:892-984) ----------------
+  size_t hx = tm.begin("extract");
+  Result* out = new Result();
+  out->n = n;
+  out->device = cx.device;
+  std::vector<int> rowmap;
+  for (int j = 0; j < nb; ++j)
+    for (int t = 0; t < ranks[j]; ++t) rowmap.push_back(j * s + t);
+  const int M = (int)rowmap.size();
+  std::vector<double> lam_f, res_f;
+  DBuf<double> Xfin;
+  int ldxf = 4;
+  std::vector<int> final_cols;
+  double t_refine = 0;
+  long long refine_steps = 0;
+  int locked_total = 0;
+  if (M > 0) {
+    std::vector<double> Th((size_t)M * M, 0.0);
+    int o = 0;
+    for (int j = 0; j < nb; ++j) {
+      const int r = ranks[j];
+      for (int c = 0; c < r; ++c)
+        for (int i = 0; i < r; ++i) Th[(size_t)(o + c) * M + o + i] = Ablk[j][(size_t)c * r + i];
+      if (j < nb - 1 && j < (int)Bblk.size()) {
+        const int rn = ranks[j + 1];
+        for (int c = 0; c < r; ++c)
+          for (int i = 0; i < rn; ++i) {
+            const double v = Bblk[j][(size_t)c * rn + i];
+            Th[(size_t)(o + c) * M + o + r + i] = v;  // T[J, I] = B
+            Th[(size_t)(o + r + i) * M + o + c] = v;  // T[I, J] = Bᵀ
+          }
+      }
+      o += r;
+    }
+    DBuf<double> Td((size_t)M * M), Dd(M), Ed(M);
+    SMALGO_HIP(hipMemcpyAsync(Td.p, Th.data(), sizeof(double) * Th.size(), hipMemcpyHostToDevice,
+                              cx.stream));
+    dn.syevd(M, Td.p, M, Dd.p, Ed.p, true);
+    std::vector<double> th(M);
+    SMALGO_HIP(hipMemcpyAsync(th.data(), Dd.p, sizeof(double) * M, hipMemcpyDeviceToHost, cx.stream));
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+    const int m = std::min(M, std::max(p.overextract * nev, nev + p.extra_keep));
+    std::vector<int> order = stable_argsort(th, true);
+    order.resize(m);
+    const int Mpad = nb * s;
+    const int ldu = round_up(m, 4);
+    DBuf<double> U((size_t)Mpad * ldu);
+    fill_block<double>(cx.stream, Mpad, ldu, U.p, ldu, 0.0);
+    DBuf<int> dsel = upload_idx(cx, order), drow = upload_idx(cx, rowmap);
+    gather_cols_colmajor(cx.stream, M, m, Td.p, M, dsel.p, drow.p, U.p, ldu);
+    // X = V U  (build_X!, polfed.jl:767-776): one MFMA GEMM instead of 64-wide chunks
+    DBuf<double> X((size_t)n * ldu);
+    gemm_nn<T, double, double>(cx.stream, n, Mpad, m, 1.0, V.p, ldV, U.p, ldu, false, 0.0, X.p, ldu);
+    gemm_flops += 2.0 * n * Mpad * m;
+    RayleighWork rw;
+    std::vector<double> lam, res;
+    rayleigh(cx, op, n, m, X.p, ldu, rw, lam, res);
+    std::vector<double> lam_keep, res_keep;
+    std::vector<int> idx_keep;
+    for (int jj = 0; jj < m; ++jj)
+      if (p.emin <= lam[jj] && lam[jj] <= p.emax) {
+        lam_keep.push_back(lam[jj]);
+        res_keep.push_back(res[jj]);
+        idx_keep.push_back(jj);
+      }
+    if (!idx_keep.empty()) {
+      const int npre = std::min((int)res_keep.size(), nev + p.extra_keep);
+      std::vector<int> ps0 = stable_argsort(res_keep);
+      ps0.resize(npre);
+      std::vector<int> idxc(npre);
+      std::vector<double> lamc(npre), resc(npre);
+      for (int i = 0; i < npre; ++i) {
+        idxc[i] = idx_keep[ps0[i]];
+        lamc[i] = lam_keep[ps0[i]];
+        resc[i] = res_keep[ps0[i]];
+      }
+      int ldc = round_up(npre, 4);
+      DBuf<double> Xc;
+      int kc = npre;
+      const bool mixed = std::is_same<T, float>::value;
+      const double anorm = std::max(std::fabs(smin), std::fabs(smax));
+      if (p.refine && mixed && p.refine_maxiter > 0) {
+        // ---- FP64 refinement of an FP32-built basis (no reference counterpart):
+        // filtered subspace iteration on the candidates plus guard vectors (the
+        // next Ritz vectors by θ), Rayleigh–Ritz in FP64, locking of pairs whose
+        // residual meets refine_tol·‖A‖ (‖A‖ = max(|smin|,|smax|)).
+        const size_t hr = tm.begin("refine");
+        const int guard = p.refine_guard >= 0 ? p.refine_guard : nev / 2;
+        std::vector<int> zc(idxc);
+        std::vector<char> used(m, 0);
+        for (int v : idxc) used[v] = 1;
+        for (int jj = 0; jj < m && (int)zc.size() < npre + guard; ++jj)
+          if (!used[jj]) zc.push_back(jj);
+        int kz = (int)zc.size();
+        int ldz = round_up(kz, 4);
+        DBuf<double> Z((size_t)n * ldz), Zb0((size_t)n * ldz), Zb1((size_t)n * ldz),
+            Lk((size_t)n * ldz), Ytmp((size_t)n * ldz), Zt((size_t)n * ldz);
+        DBuf<int> dz = upload_idx(cx, zc);
+        gather_cols_rm<double>(cx.stream, n, kz, X.p, ldu, dz.p, Z.p, ldz);
+        const int Kr = p.refine_degree >= 0 ? p.refine_degree : degree;
+        const std::vector<double> cr =
+            build_coeffs(Kr, smin, smax, p.sigma, p.emin, p.emax, p.window_type, p.jackson != 0);
+        const double tol_abs = p.refine_tol * anorm;
+        RRWork rr;
+        int nlock = 0;
+        std::vector<double> lam_lock, res_lock;
+        std::vector<double> lam_act, res_act;
+        OrthWork ow2;
+        DBuf<double> Cl, glw;
+        DBuf<double> pb2;
+        if (op.is_callback()) pb2.alloc((size_t)n * ldz);
+        for (int it = 0; it < p.refine_maxiter && kz > 0; ++it) {
+          // Y = p_r(Â) Z
+          apply_filter<double>(cx, op, kz, Z.p, ldz, cr, smin, smax, Ytmp.p, ldz, Zb0.p, Zb1.p, ldz,
+                               pb2.p);
+          refine_steps += (long long)Kr * kz;
+          // project out the locked vectors
+          if (nlock > 0) {
+            Cl.ensure((size_t)nlock * ldz);
+            glw.ensure(gemm_tn_work_elems(n, nlock, kz));
+            gemm_tn<double, double, double>(cx.stream, n, nlock, kz, 1.0, Lk.p, ldz, Ytmp.p, ldz,
+                                            0.0, Cl.p, ldz, glw.p, glw.n);
+            gemm_nn<double, double, double>(cx.stream, n, nlock, kz, -1.0, Lk.p, ldz, Cl.p, ldz,
+                                            false, 1.0, Ytmp.p, ldz);
+          }
+          DBuf<double> Yr;
+          int ldyr = 4;
+          const int kq = refine_rr(cx, dn, op, n, kz, Ytmp.p, ldz, rr, Yr, ldyr, lam_act, res_act);
+          if (kq == 0) break;
+          // lock converged inside pairs
+          std::vector<int> keep, lock;
+          int inside_unconv = 0;
+          for (int i = 0; i < kq; ++i) {
+            const bool inside = p.emin <= lam_act[i] && lam_act[i] <= p.emax;
+            if (inside && res_act[i] <= tol_abs && nlock + (int)lock.size() < ldz)
+              lock.push_back(i);
+            else {
+              keep.push_back(i);
+              if (inside) ++inside_unconv;
+            }
+          }
+          if (!lock.empty()) {
+            DBuf<int> dl = upload_idx(cx, lock);
+            gather_cols_rm<double>(cx.stream, n, (int)lock.size(), Yr.p, ldyr, dl.p, Lk.p + nlock,
+                                   ldz);
+            for (int i : lock) {
+              lam_lock.push_back(lam_act[i]);
+              res_lock.push_back(res_act[i]);
+            }
+            nlock += (int)lock.size();
+          }
+          std::vector<double> la2, re2;
+          for (int i : keep) {
+            la2.push_back(lam_act[i]);
+            re2.push_back(res_act[i]);
+          }
+          lam_act.swap(la2);
+          res_act.swap(re2);
+          kz = (int)keep.size();
+          if (kz > 0) {
+            DBuf<int> dk = upload_idx(cx, keep);
+            gather_cols_rm<double>(cx.stream, n, kz, Yr.p, ldyr, dk.p, Z.p, ldz);
+          }
+          SMALGO_HIP(hipStreamSynchronize(cx.stream));
+          if (inside_unconv == 0) break;
+        }
+        locked_total = nlock;
+        // candidate set = locked ∪ active; the reference's post-refine selection follows
+        kc = nlock + kz;
+        ldc = round_up(std::max(kc, 1), 4);
+        Xc.alloc((size_t)n * ldc);
+        if (nlock) copy_block<double, double>(cx.stream, n, nlock, Lk.p, ldz, Xc.p, ldc);
+        if (kz) copy_block<double, double>(cx.stream, n, kz, Z.p, ldz, Xc.p + nlock, ldc);
+        lamc = lam_lock;
+        resc = res_lock;
+        lamc.insert(lamc.end(), lam_act.begin(), lam_act.end());
+        resc.insert(resc.end(), res_act.begin(), res_act.end());
+        // sort the candidates by λ like refine_original_rr's output
+        std::vector<int> pl = stable_argsort(lamc);
+        std::vector<double> l2(kc), r2(kc);
+        for (int i = 0; i < kc; ++i) {
+          l2[i] = lamc[pl[i]];
+          r2[i] = resc[pl[i]];
+        }
+        DBuf<int> dpl = upload_idx(cx, pl);
+        DBuf<double> Xs((size_t)n * ldc);
+        gather_cols_rm<double>(cx.stream, n, kc, Xc.p, ldc, dpl.p, Xs.p, ldc);
+        Xc = std::move(Xs);
+        lamc.swap(l2);
+        resc.swap(r2);
+        SMALGO_HIP(hipStreamSynchronize(cx.stream));
+        tm.end(hr);
+      } else {
+        Xc.alloc((size_t)n * ldc);
+        DBuf<int> dc = upload_idx(cx, idxc);
+        gather_cols_rm<double>(cx.stream, n, npre, X.p, ldu, dc.p, Xc.p, ldc);
+        if (p.refine) {
+          const size_t hr = tm.begin("refine");
+          RRWork rr;
+          DBuf<double> Yr;
+          int ldyr = 4;
+          kc = refine_rr(cx, dn, op, n, npre, Xc.p, ldc, rr, Yr, ldyr, lamc, resc);
+          Xc = std::move(Yr);
+          ldc = ldyr;
+          tm.end(hr);
+        }
+      }
+      // inside2 filter (polfed.jl:971-975) — the unrefined branch already holds
+      // only inside pairs
+      std::vector<int> ins;
+      for (int i = 0; i < kc; ++i)
+        if (p.emin <= lamc[i] && lamc[i] <= p.emax) ins.push_back(i);
+      if (!ins.empty()) {
+        std::vector<double> l2, r2;
+        for (int i : ins) {
+          l2.push_back(lamc[i]);
+          r2.push_back(resc[i]);
+        }
+        const int nkeep = std::min((int)r2.size(), nev + p.extra_keep);
+        std::vector<int> ps = stable_argsort(r2);
+        ps.resize(nkeep);
+        std::vector<double> lf(nkeep), rf(nkeep);
+        std::vector<int> cf(nkeep);
+        for (int i = 0; i < nkeep; ++i) {
+          lf[i] = l2[ps[i]];
+          rf[i] = r2[ps[i]];
+          cf[i] = ins[ps[i]];
+        }
+        std::vector<int> pe = stable_argsort(lf);
+        for (int i = 0; i < nkeep; ++i) {
+          lam_f.push_back(lf[pe[i]]);
+          res_f.push_back(rf[pe[i]]);
+          final_cols.push_back(cf[pe[i]]);
+        }
+        Xfin = std::move(Xc);
+        ldxf = ldc;
+      }
+    }
+  }
+  tm.end(hx);
+  out->line = progress_line(nb, lam_f, res_f);
+  if (p.show_progress) {
+    std::printf("%s\n", out->line.c_str());
+    std::fflush(stdout);
+  }
+  // ---- return rule (polfed.jl:1148-1157)
+  std::vector<int> pick;
+  if (!res_f.empty() && (int)res_f.size() >= nev) {
+    std::vector<double> rs(res_f);
+    std::sort(rs.begin(), rs.end());
+    if (rs[nev - 1] < p.tol)
+      for (int i = 0; i < nev; ++i) pick.push_back(i);
+  }
+  if (pick.empty() && !res_f.empty()) {
+    pick = stable_argsort(res_f);
+    pick.resize(std::min(nev, (int)res_f.size()));
+  }
+  const int kout = (int)pick.size();
+  std::vector<int> cols(kout);
+  for (int i = 0; i < kout; ++i) {
+    out->lam.push_back(lam_f[pick[i]]);
+    out->res.push_back(res_f[pick[i]]);
+    cols[i] = final_cols[pick[i]];
+  }
+  if (kout > 0) {
+    SMALGO_HIP(hipMalloc((void**)&out->Xdev, sizeof(double) * (size_t)n * kout));
+    DBuf<int> dc = upload_idx(cx, cols);
+    rm_to_cm<double>(cx.stream, n, kout, Xfin.p, ldxf, dc.p, out->Xdev, n);
+  }
+  auto ph = tm.resolve();
+  const double t_total = std::chrono::duration<double>(clk::now() - t0).count();
+  t_filter = ph["filter"];
+  t_refine = ph["refine"];
+  out->stats["t_total"] = t_total;
+  out->stats["t_filter"] = t_filter;
+  out->stats["t_orth"] = ph["orth"];
+  out->stats["t_extract"] = ph["extract"];
+  out->stats["t_refine"] = t_refine;
+  out->stats["filter_steps"] = (double)filter_steps;
+  out->stats["block_steps"] = nb;
+  out->stats["basis_dim"] = M;
+  out->stats["iterations"] = nb;
+  out->stats["refine_filter_steps"] = (double)refine_steps;
+  out->stats["locked"] = locked_total;
+  out->stats["gemm_flops"] = gemm_flops;
+  out->stats["smin"] = smin;
+  out->stats["smax"] = smax;
+  // algorithmic bytes of the basis-build filter: per fused step
+  // nnz*(w+4) + 4(n+1) + 4*n*b*w   (b = block width, w = sizeof(T))
+  double bytes = 0;
+  for (int j = 0; j < nb; ++j)
+    bytes += (double)degree * ((double)(op.A ? op.A->nnz : 0) * (sizeof(T) + 4) + 4.0 * (n + 1) +
+                               4.0 * n * ranks[j] * sizeof(T));
+  out->stats["spmm_bytes"] = bytes;
+  return out;
+}
+
+Result* polfed_solve(Ctx& cx, const Op& op, int nev, int degree, const smalgo_polfed_params& p,
+                     const double* Q1) {
+  if (p.precision == SMALGO_PREC_FP32_REFINE64) {
+    if (op.is_callback()) throw Error(-1, "matrix-free POLFED runs in FP64 only");
+    return polfed_run<float>(cx, op, nev, degree, p, Q1);
+  }
+  return polfed_run<double>(cx, op, nev, degree, p, Q1);
+}
+
+// ---------------------------------------------------------------------------
+// Chebyshev filtered subspace iteration (FP64)
+// ---------------------------------------------------------------------------
+Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const smalgo_cheb_params& p,
+                        const double* Q0, bool matrix_free) {
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  const int n = op.n;
+  if (!(emin < emax)) throw Error(-1, "need emin<emax");
+  if (!(p.m0 > 0)) throw Error(-1, "m0 must be positive");
+  if (!(p.degree >= 0)) throw Error(-1, "degree must be nonnegative");
+  const int m0 = std::min(p.m0, n);
+  double smin = p.smin, smax = p.smax;
+  if (!p.has_bounds) {
+    if (op.is_callback())
+      throw Error(-1, "gershgorin_bounds has no matrix-free method; provide smin and smax");
+    smin = op.A->gersh_min;
+    smax = op.A->gersh_max;
+  }
+  Result* out = new Result();
+  out->n = n;
+  out->device = cx.device;
+  if (!(emin >= smin && emax <= smax)) {
+    out->stats["warn_interval"] = 1.0;
+    if (!matrix_free)
+      std::fprintf(stderr,
+                   "Warning: target interval not contained in supplied spectral bounds; filter "
+                   "quality may degrade\n");
+  }
+  const double c = 0.5 * (smin + smax), d = 0.5 * (smax - smin);
+  const double a = (emin - c) / d, b = (emax - c) / d;
+  if (!(-1.0 <= a && a < b && b <= 1.0)) {
+    delete out;
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "mapped interval [%s,%s] not inside [-1,1]%s",
+                  format_julia_float(a).c_str(), format_julia_float(b).c_str(),
+                  matrix_free ? "" : "; provide better spectral bounds");
+    throw Error(-1, buf);
+  }
+  const double sh = (p.sigma - c) / d;
+  std::vector<double> coeffs;
+  switch (p.window_type) {
+    case SMALGO_WINDOW_STEP: coeffs = cheb_window_coeffs(a, b, p.degree, p.jackson != 0); break;
+    case SMALGO_WINDOW_DELTA: coeffs = cheb_delta_coeffs(sh, p.degree, p.jackson != 0); break;
+    case SMALGO_WINDOW_SMOOTH:
+      coeffs = cheb_smooth_window_coeffs(a, b, p.degree, p.jackson != 0, 4096, 0.02);
+      break;
+    default:
+      delete out;
+      throw Error(-1, "Unknown window type, possible are :step, :delta, and :smooth");
+  }
+  const int ld = round_up(m0, 4);
+  PhaseTimer tm(cx.stream);
+  Dense dn(cx);
+  OrthWork ow;
+  DBuf<double> Q((size_t)n * ld), Y((size_t)n * ld), B0((size_t)n * ld), B1((size_t)n * ld),
+      X((size_t)n * ld), AX((size_t)n * ld), Aq((size_t)m0 * m0), wv(m0), ev(m0), lamd(m0),
+      resd(m0), red(colred_work_elems(n, m0)), gw(gemm_tn_work_elems(n, m0, m0));
+  DBuf<double> pbuf;
+  if (op.is_callback()) pbuf.alloc((size_t)n * ld);
+  if (Q0) {
+    DBuf<double> cm((size_t)n * m0);
+    SMALGO_HIP(hipMemcpyAsync(cm.p, Q0, sizeof(double) * (size_t)n * m0, hipMemcpyHostToDevice,
+                              cx.stream));
+    cm_to_rm<double, double>(cx.stream, n, m0, cm.p, n, Q.p, ld);
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  } else {
+    gaussian_block<double>(cx.stream, n, m0, p.seed, Q.p, ld);
+  }
+  std::vector<double> lam(m0, 0.0), res(m0, std::numeric_limits<double>::infinity());
+  bool prev_ok = false;
+  int its = 0;
+  long long filter_steps = 0;
+  for (int it = 1; it <= p.maxiter; ++it) {
+    its = it;
+    size_t hf = tm.begin("filter");
+    apply_filter<double>(cx, op, m0, Q.p, ld, coeffs, smin, smax, Y.p, ld, B0.p, B1.p, ld, pbuf.p);
+    tm.end(hf);
+    filter_steps += p.degree;
+    size_t ho = tm.begin("orth");
+    // Q .= Y; thin_orth!(Q)  — all min(n,m0) columns kept (:733-734)
+    cholqr<double>(cx, dn, ow, n, m0, Y.p, ld, Q.p, ld, B0.p, ld, 0.0, nullptr);
+    tm.end(ho);
+    size_t hr = tm.begin("rr");
+    apply_A<double>(cx, op, m0, Q.p, ld, AX.p, ld);                       // R = A Q
+    gemm_tn<double, double, double>(cx.stream, n, m0, m0, 1.0, Q.p, ld, AX.p, ld, 0.0, Aq.p, m0,
+                                    gw.p, gw.n);                          // Aq = QᵀR
+    // eigen!(Symmetric(Aq)) reads the upper triangle of Aq; Aq is row-major
+    // here, i.e. the lower triangle of its column-major view.
+    dn.syevd(m0, Aq.p, m0, wv.p, ev.p, false);
+    gemm_nn<double, double, double>(cx.stream, n, m0, m0, 1.0, Q.p, ld, Aq.p, m0, true, 0.0, X.p,
+                                    ld);                                  // X = Q Xq
+    apply_A<double>(cx, op, m0, X.p, ld, AX.p, ld);
+    col_resid<double>(cx.stream, n, m0, X.p, ld, AX.p, ld, wv.p, resd.p, red.p, red.n);
+    SMALGO_HIP(hipMemcpyAsync(lam.data(), wv.p, sizeof(double) * m0, hipMemcpyDeviceToHost,
+                              cx.stream));
+    SMALGO_HIP(hipMemcpyAsync(res.data(), resd.p, sizeof(double) * m0, hipMemcpyDeviceToHost,
+                              cx.stream));
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+    tm.end(hr);
+    int cnt = 0;
+    double maxres = -1;
+    for (int j = 0; j < m0; ++j)
+      if (lam[j] >= emin && lam[j] <= emax && res[j] < p.res_gate) {
+        ++cnt;
+        maxres = std::max(maxres, res[j]);
+      }
+    if (cnt == 0) maxres = std::numeric_limits<double>::infinity();
+    if (p.debug) {
+      std::printf("it=%d inside=%d maxres=%.3e\n", it, cnt, maxres);
+      std::fflush(stdout);
+    }
+    if (matrix_free) {  // :827-836
+      const bool ok = cnt > 0 && maxres < p.tol;
+      if (p.two_hit) {
+        if (ok && (it == 1 || prev_ok)) break;
+        prev_ok = ok;
+      } else if (ok) {
+        break;
+      }
+    }
+  }
+  // selection (:750-756)
+  std::vector<int> idx;
+  std::vector<double> ls;
+  const double gate = std::max(p.res_gate, 10 * p.tol);
+  for (int j = 0; j < m0; ++j)
+    if (lam[j] >= emin && lam[j] <= emax && res[j] < gate) {
+      idx.push_back(j);
+      ls.push_back(lam[j]);
+    }
+  std::vector<int> pp = stable_argsort(ls);
+  std::vector<int> cols;
+  for (int i : pp) {
+    cols.push_back(idx[i]);
+    out->lam.push_back(lam[idx[i]]);
+    out->res.push_back(res[idx[i]]);
+  }
+  if (!cols.empty()) {
+    SMALGO_HIP(hipMalloc((void**)&out->Xdev, sizeof(double) * (size_t)n * cols.size()));
+    DBuf<int> dc = upload_idx(cx, cols);
+    rm_to_cm<double>(cx.stream, n, (int)cols.size(), X.p, ld, dc.p, out->Xdev, n);
+  }
+  auto ph = tm.resolve();
+  out->stats["t_total"] = std::chrono::duration<double>(clk::now() - t0).count();
+  out->stats["t_filter"] = ph["filter"];
+  out->stats["t_orth"] = ph["orth"];
+  out->stats["t_extract"] = ph["rr"];
+  out->stats["iterations"] = its;
+  out->stats["filter_steps"] = (double)filter_steps;
+  out->stats["smin"] = smin;
+  out->stats["smax"] = smax;
+  out->stats["spmm_bytes"] = (double)filter_steps * ((double)(op.A ? op.A->nnz : 0) * 12.0 +
+                                                    4.0 * (n + 1) + 4.0 * n * m0 * 8.0);
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "it=%d inside=%zu", its, out->lam.size());
+  out->line = buf;
+  return out;
+}
+
+}  // namespace smalgo

@@ -1,0 +1,135 @@
+// Host drivers: context, operators, POLFED and Chebyshev subspace iteration.
+#pragma once
+
+#include <rocblas/rocblas.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/smalgo.h"
+#include "smalgo_internal.h"
+
+namespace smalgo {
+
+// RAII device buffer
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  explicit DBuf(size_t count) { alloc(count); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  void alloc(size_t count) {
+    release();
+    if (count) SMALGO_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void ensure(size_t count) { if (count > n) alloc(count); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  std::string err;
+  std::map<long long, FilterTile> tuned;  // (precision, b, matrix id) -> tile
+};
+
+struct Mat {
+  Ctx* ctx = nullptr;
+  DevCSR A;
+  int symmetric = -1;  // computed at upload: 1 / 0
+  long long id = 0;
+};
+
+// Operator: device CSR or host callback (Y <- A X on column-major host arrays)
+struct Op {
+  const DevCSR* A = nullptr;
+  smalgo_matblock_fn fn = nullptr;
+  void* user = nullptr;
+  int n = 0;
+  long long mat_id = 0;
+  bool is_callback() const { return fn != nullptr; }
+};
+
+struct Result {
+  int64_t n = 0;
+  std::vector<double> lam, res;
+  double* Xdev = nullptr;  // column-major n x k on the device
+  int device = 0;
+  std::map<std::string, double> stats;
+  std::string line;
+  ~Result() {
+    if (Xdev) (void)hipFree(Xdev);
+  }
+};
+
+// Small FP64 dense LAPACK-type steps on the device (rocSOLVER).
+struct Dense {
+  Ctx& cx;
+  DBuf<int> info;
+  explicit Dense(Ctx& c) : cx(c), info(1) {}
+  int potrf_upper(int b, double* A, int lda);  // returns info (0 = success)
+  void trtri_upper(int b, double* A, int lda);
+  void syevd(int n, double* A, int lda, double* w, double* work_e, bool upper);
+};
+
+// Timing of device phases with events on the context stream.
+struct PhaseTimer {
+  hipStream_t st;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  std::map<std::string, double> open_;
+  explicit PhaseTimer(hipStream_t s) : st(s) {}
+  size_t begin(const std::string& name);
+  void end(size_t h);
+  std::map<std::string, double> resolve();  // seconds per phase (syncs)
+  ~PhaseTimer();
+};
+
+FilterTile choose_tile(size_t elem, int b, std::initializer_list<const void*> ptrs,
+                       std::initializer_list<int> lds);
+
+// block Clenshaw filter Y = p(Â) Q (coefficients on the host)
+template <typename T>
+void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::vector<double>& co,
+                  double smin, double smax, T* Y, int ldy, T* buf0, T* buf1, int ldb,
+                  double* pbuf /* callback staging, n x ldb doubles, may be null for CSR */);
+
+// Y = A X (CSR or callback)
+template <typename T>
+void apply_A(Ctx& cx, const Op& op, int b, const T* X, int ldx, T* Y, int ldy);
+
+Result* polfed_solve(Ctx& cx, const Op& op, int nev, int degree, const smalgo_polfed_params& p,
+                     const double* Q1);
+Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const smalgo_cheb_params& p,
+                        const double* Q0, bool matrix_free);
+
+// One-call versions of the reference's block kernels (host column-major arrays).
+void orth_against_host(Ctx& cx, int n, int b, double* W, int ldw, int m, const double* Q, int ldq,
+                       int precision);
+void block_qr_host(Ctx& cx, int n, int b, const double* W, int ldw, double rtol, int precision,
+                   double* Q, int ldq, double* R, int ldr, int* rank);
+void build_X_host(Ctx& cx, int n, int M, int k, const double* V, int ldv, const double* U, int ldu,
+                  double* X, int ldx);
+void rayleigh_host(Ctx& cx, const Op& op, int k, const double* X, int ldx, double* lam, double* res);
+void residuals_host(Ctx& cx, const Op& op, int k, const double* X, int ldx, const double* lam,
+                    double* res);
+Result* refine_rr_host(Ctx& cx, const Op& op, int k, const double* X, int ldx);
+
+std::string format_julia_float(double v);
+std::string progress_line(int m, const std::vector<double>& lam, const std::vector<double>& res);
+
+}  // namespace smalgo
