@@ -140,6 +140,15 @@ int smalgo_refine_original_rr(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matbl
                               void* user, int64_t n, int64_t k, const double* X, int64_t ldx,
                               smalgo_result** out);
 
+/* ---- kernel polynomial method (no reference counterpart; sizes intervals and
+ * feeds the learned filter parameters) ----------------------------------
+ * mu[k] = (1/(R·n)) Σ_r r_rᵀ T_k(Â) r_r, k = 0..nmom-1, with Â=(A-cI)/d,
+ * c=(smin+smax)/2, d=(smax-smin)/2, over R Gaussian probes drawn from seed
+ * (splitmix64/Box-Muller).  Uses the same fused SpMM kernel as the filter and
+ * the doubling identities μ_2k = 2⟨T_k,T_k⟩-μ_0, μ_2k+1 = 2⟨T_k+1,T_k⟩-μ_1. */
+int smalgo_kpm_moments(smalgo_ctx* ctx, const smalgo_mat* A, int nmom, int R, uint64_t seed,
+                       double smin, double smax, double* mu);
+
 /* ---- POLFED ---------------------------------------------------------------
  * Keyword arguments of polfed(A::MatrixFreeRealSymOp,nev,degree;...)
  * (polfed.jl:1128) plus the mixed-precision controls.  has_* flags stand in

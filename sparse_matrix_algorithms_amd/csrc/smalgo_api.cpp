@@ -361,6 +361,18 @@ int smalgo_refine_original_rr(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matbl
   });
 }
 
+int smalgo_kpm_moments(smalgo_ctx* ctx, const smalgo_mat* A, int nmom, int R, uint64_t seed,
+                       double smin, double smax, double* mu) {
+  if (!ctx || !A || !mu) return SMALGO_ERR_ARG;
+  return guarded(ctx, [&] {
+    Op op;
+    op.A = &A->A;
+    op.n = A->A.n;
+    op.mat_id = A->id;
+    kpm_moments(*ctx, op, nmom, R, seed, smin, smax, mu);
+  });
+}
+
 // ---- drivers ---------------------------------------------------------------
 void smalgo_polfed_default_params(smalgo_polfed_params* p) {
   if (!p) return;

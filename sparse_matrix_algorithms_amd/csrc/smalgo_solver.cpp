@@ -570,6 +570,61 @@ Result* refine_rr_host(Ctx& cx, const Op& op, int k, const double* X, int ldx) {
 }
 
 // ---------------------------------------------------------------------------
+// KPM moments with the doubling identities (Weiße et al., Rev. Mod. Phys. 78,
+// 275 (2006), Sec. II.B): per Chebyshev step one fused SpMM of the R probes.
+// ---------------------------------------------------------------------------
+void kpm_moments(Ctx& cx, const Op& op, int nmom, int R, uint64_t seed, double smin, double smax,
+                 double* mu) {
+  const int n = op.n;
+  if (nmom < 2 || R < 1) throw Error(-1, "need nmom >= 2 and R >= 1");
+  if (!(smax > smin)) throw Error(-1, "need smin < smax");
+  if (op.is_callback()) throw Error(-1, "KPM runs on device matrices");
+  const int ld = round_up(R, 4);
+  const double c = 0.5 * (smin + smax), invd = 2.0 / (smax - smin);
+  DBuf<double> r0((size_t)n * ld), ta((size_t)n * ld), tb((size_t)n * ld);
+  const int kmax = (nmom + 1) / 2;  // T_0 .. T_kmax needed
+  DBuf<double> dots((size_t)(2 * kmax + 2) * ld), red(colred_work_elems(n, R));
+  gaussian_block<double>(cx.stream, n, R, seed, r0.p, ld);
+  const FilterTile t = choose_tile(sizeof(double), R, {r0.p, ta.p, tb.p}, {ld});
+  // T_1 = Â r
+  launch_clenshaw_step<double>(cx.stream, *op.A, R, nullptr, 0, 0.0, r0.p, ld, invd, -c * invd,
+                               nullptr, 0, 0.0, ta.p, ld, t);
+  // slot 2k: <T_k,T_k>, slot 2k+1: <T_k+1,T_k>
+  col_dots<double>(cx.stream, n, R, r0.p, ld, r0.p, ld, dots.p + 0 * ld, red.p, red.n);
+  col_dots<double>(cx.stream, n, R, ta.p, ld, r0.p, ld, dots.p + 1 * ld, red.p, red.n);
+  double* Tk = ta.p;        // T_k   (k = 1)
+  double* Tkm1 = r0.p;      // T_k-1
+  double* spare = tb.p;
+  for (int k = 1; k < kmax; ++k) {
+    // T_k+1 = 2Â T_k − T_k-1  (written over T_k-1 unless that is the probe block)
+    double* out = (Tkm1 == r0.p) ? spare : Tkm1;
+    launch_clenshaw_step<double>(cx.stream, *op.A, R, nullptr, 0, 0.0, Tk, ld, 2 * invd,
+                                 -2 * c * invd, Tkm1, ld, 1.0, out, ld, t);
+    col_dots<double>(cx.stream, n, R, Tk, ld, Tk, ld, dots.p + (size_t)(2 * k) * ld, red.p, red.n);
+    col_dots<double>(cx.stream, n, R, out, ld, Tk, ld, dots.p + (size_t)(2 * k + 1) * ld, red.p,
+                     red.n);
+    Tkm1 = Tk;
+    Tk = out;
+  }
+  std::vector<double> h((size_t)(2 * kmax) * ld);
+  SMALGO_HIP(hipMemcpyAsync(h.data(), dots.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                            cx.stream));
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  auto avg = [&](int slot) {
+    double s = 0;
+    for (int r = 0; r < R; ++r) s += h[(size_t)slot * ld + r];
+    return s / ((double)R * n);
+  };
+  const double m0 = avg(0), m1 = avg(1);
+  mu[0] = m0;
+  if (nmom > 1) mu[1] = m1;
+  for (int k = 1; k < kmax; ++k) {
+    if (2 * k < nmom) mu[2 * k] = 2 * avg(2 * k) - m0;
+    if (2 * k + 1 < nmom) mu[2 * k + 1] = 2 * avg(2 * k + 1) - m1;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // POLFED
 // ---------------------------------------------------------------------------
 template <typename T>

@@ -129,6 +129,9 @@ void residuals_host(Ctx& cx, const Op& op, int k, const double* X, int ldx, cons
                     double* res);
 Result* refine_rr_host(Ctx& cx, const Op& op, int k, const double* X, int ldx);
 
+void kpm_moments(Ctx& cx, const Op& op, int nmom, int R, uint64_t seed, double smin, double smax,
+                 double* mu);
+
 std::string format_julia_float(double v);
 std::string progress_line(int m, const std::vector<double>& lam, const std::vector<double>& res);
 

@@ -1,0 +1,85 @@
+"""Kernel polynomial method on the GPU: moments, density of states, stochastic
+eigenvalue counts and interval sizing (no reference counterpart; the
+reference requires the interval and the degree from the caller,
+``polfed.jl:1128``).
+
+Moments come from ``smalgo_kpm_moments`` (same fused SpMM kernel as the
+filter).  With ``μ_k = Tr T_k(Â)/n`` the Jackson-damped expansions are
+(Weiße et al., Rev. Mod. Phys. 78, 275 (2006), Eqs. 71, 82):
+
+    ρ(x)     = [g_0 μ_0 + 2 Σ_k g_k μ_k T_k(x)] / (π √(1−x²))
+    N[a, b]  = n Σ_k c_k μ_k,   c_k = cheb_window_coeffs(a, b, K)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from .solvers import Context, DeviceMatrix, cheb_window_coeffs, default_context, jackson_kernel
+
+_DP = C.POINTER(C.c_double)
+
+
+def kpm_moments(A, nmom: int, R: int = 16, seed: int = 7, smin: Optional[float] = None,
+                smax: Optional[float] = None, ctx: Optional[Context] = None):
+    """Return ``(μ, smin, smax)``: ``nmom`` normalised Chebyshev moments of
+    ``Â=(A−cI)/d`` from ``R`` Gaussian probes (bounds default to Gershgorin)."""
+    ctx = ctx or default_context()
+    dm = A if isinstance(A, DeviceMatrix) else DeviceMatrix(A, ctx)
+    if smin is None or smax is None:
+        smin, smax = dm.gershgorin_bounds()
+    mu = np.zeros(nmom)
+    ctx.check(ctx.lib.smalgo_kpm_moments(ctx.h, dm.h, nmom, R, seed, smin, smax,
+                                         mu.ctypes.data_as(_DP)))
+    return mu, smin, smax
+
+
+def dos(mu: np.ndarray, x: np.ndarray, jackson: bool = True) -> np.ndarray:
+    """Jackson-damped density of states per unit ``x`` in ``(−1, 1)`` (integrates to μ_0)."""
+    K = len(mu) - 1
+    g = jackson_kernel(K) if jackson else np.ones(K + 1)
+    x = np.asarray(x, dtype=np.float64)
+    th = np.arccos(np.clip(x, -1, 1))
+    k = np.arange(1, K + 1)
+    s = g[0] * mu[0] + 2.0 * np.sum((g[1:] * mu[1:])[None, :] * np.cos(np.outer(th, k)), axis=1)
+    return s / (math.pi * np.sqrt(np.maximum(1e-300, 1 - x * x)))
+
+
+def count(mu: np.ndarray, n: int, emin: float, emax: float, smin: float, smax: float) -> float:
+    """Stochastic estimate of the number of eigenvalues in ``[emin, emax]``."""
+    c0 = 0.5 * (smin + smax)
+    d = 0.5 * (smax - smin)
+    co = cheb_window_coeffs((emin - c0) / d, (emax - c0) / d, len(mu) - 1, True)
+    return float(n * np.dot(co, mu))
+
+
+def interval_for_count(mu: np.ndarray, n: int, target: float, smin: float, smax: float,
+                       center: float = 0.0):
+    """Symmetric interval ``[center−δ, center+δ]`` whose KPM count equals ``target``."""
+    lo, hi = 0.0, 0.5 * (smax - smin)
+    for _ in range(100):
+        mid = 0.5 * (lo + hi)
+        if count(mu, n, center - mid, center + mid, smin, smax) < target:
+            lo = mid
+        else:
+            hi = mid
+    delta = 0.5 * (lo + hi)
+    return center - delta, center + delta
+
+
+def plain_parameters(n: int, nev: int, emin: float, emax: float, smin: float, smax: float,
+                     s: int, kappa: float = 8.0, mmax_factor: float = 2.8):
+    """The project's "plain" POLFED parameter rule (the reference takes the
+    degree from the caller and defaults ``mmax=ceil(2.8·nev/s)``, ``polfed.jl:1132``):
+    degree ``K = ceil(κ / h)`` with ``h`` the half-width of the target window in
+    the Chebyshev variable, i.e. the step filter resolves the window with κ
+    Jackson widths; ``mmax = ceil(mmax_factor · nev / s)``."""
+    d = 0.5 * (smax - smin)
+    h = 0.5 * (emax - emin) / d
+    K = int(math.ceil(kappa / h))
+    mmax = int(math.ceil(mmax_factor * nev / s))
+    return K, mmax
