@@ -149,6 +149,12 @@ int smalgo_refine_original_rr(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matbl
 int smalgo_kpm_moments(smalgo_ctx* ctx, const smalgo_mat* A, int nmom, int R, uint64_t seed,
                        double smin, double smax, double* mu);
 
+/* ---- measurement helper (no reference counterpart) -----------------------
+ * Runs `reps` device-to-device copies of `bytes` bytes with a plain 16-byte
+ * vector grid-stride copy kernel (stream_copy_kernel) on the context stream and returns the mean seconds per
+ * copy; used to calibrate HBM byte counters (FETCH_SIZE / WRITE_SIZE). */
+int smalgo_copy_bench(smalgo_ctx* ctx, int64_t bytes, int reps, double* seconds);
+
 /* ---- POLFED ---------------------------------------------------------------
  * Keyword arguments of polfed(A::MatrixFreeRealSymOp,nev,degree;...)
  * (polfed.jl:1128) plus the mixed-precision controls.  has_* flags stand in

@@ -89,6 +89,7 @@ SIGNATURES = [
      [_P, _P, MATBLOCK_FN, _P, C.c_int64, C.c_int64, _DP, C.c_int64, C.POINTER(_P)]),
     ("smalgo_kpm_moments", C.c_int,
      [_P, _P, C.c_int, C.c_int, C.c_uint64, C.c_double, C.c_double, _DP]),
+    ("smalgo_copy_bench", C.c_int, [_P, C.c_int64, C.c_int, _DP]),
     ("smalgo_polfed_default_params", None, [C.POINTER(PolfedParams)]),
     ("smalgo_polfed", C.c_int,
      [_P, _P, MATBLOCK_FN, _P, C.c_int64, C.c_int, C.c_int, C.POINTER(PolfedParams), _DP,

@@ -53,7 +53,8 @@ template <> inline const float* DevCSR::vals<float>() const { return val32; }
 // ---------------------------------------------------------------------------
 struct FilterTile {
   int vec = 1;   // elements per lane per load
-  int lpr = 64;  // lanes per row
+  int lpr = 64;  // lanes per row (rows per wavefront = 64/lpr)
+  int rt = 1;    // consecutive rows per lane team
 };
 
 template <typename T>
@@ -67,6 +68,12 @@ template <typename T>
 void launch_clenshaw_step_cols(hipStream_t st, const DevCSR& A, int b, const T* Q, int ldq,
                                const T* cq, const T* X1, int ld1, const T* a1, const T* b1,
                                const T* X2, int ld2, const T* s2, T* out, int ldo, FilterTile tile);
+
+// Panel layout for wide blocks: ceil(b/PW) dense n x PW slabs (slab stride n*PW).
+template <typename T>
+void to_panels(hipStream_t st, int n, int b, int pw, const T* S, int lds, T* D);
+template <typename T>
+void from_panels(hipStream_t st, int n, int b, int pw, const T* S, T* D, int ldd);
 
 // ---------------------------------------------------------------------------
 // tall-skinny MFMA GEMMs
@@ -127,6 +134,8 @@ void launch_clenshaw_step_pre(hipStream_t st, int n, int b, const double* Q, int
                               const double* P, int ldp, const double* X1, int ld1, double a1,
                               double b1, const double* X2, int ld2, double s2, double* out,
                               int ldo);
+// 16-byte-vector grid-stride copy of `bytes` (multiple of 16)
+void stream_copy(hipStream_t st, const void* src, void* dst, size_t bytes);
 // number of compute units, queried once per device
 int device_cu_count();
 

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace smalgo {
 
@@ -65,6 +66,16 @@ __device__ __forceinline__ int xcd_tile(int per_xcd) {
   return (w & 7) * per_xcd + (w >> 3);
 }
 
+// SMALGO_XCD_REMAP=0 turns the remapping off (tile = workgroup id) for A/B tests
+static bool xcd_remap_enabled() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("SMALGO_XCD_REMAP");
+    v = (e && e[0] == '0') ? 0 : 1;
+  }
+  return v == 1;
+}
+
 static inline int grid_xcd(int ntiles, int* per_xcd) {
   const int g = (ntiles + 7) / 8 * 8;
   *per_xcd = g / 8;
@@ -74,121 +85,156 @@ static inline int grid_xcd(int ntiles, int* per_xcd) {
 // ---------------------------------------------------------------------------
 // fused Clenshaw step
 //   out = cq*Q + a1*(A X1 or P) + b1*X1 - s2*X2
-// One row per team of LPR lanes; each lane covers V consecutive columns and
-// strides by LPR*V over the block width.  64/LPR rows per wavefront, four
-// wavefronts per workgroup.
+// A team of LPR lanes owns one row of one column chunk of width LPR*V; each
+// lane covers V consecutive columns; 64/LPR rows per wavefront, four
+// wavefronts per workgroup.  Tiles run column-chunk-major, so with the
+// XCD-aware mapping every XCD streams a contiguous row range of one column
+// chunk and keeps the rows its CSR neighbours gather within its own L2.
 // ---------------------------------------------------------------------------
-template <typename T, int V, int LPR, bool PERCOL>
+template <typename T, int V, int LPR, int RT, bool PERCOL>
 __global__ __launch_bounds__(256) void clenshaw_kernel(
-    int n, int b, int ntiles, int per_xcd, const int* __restrict__ rowptr,
+    int n, int b, int nrowtiles, int ntiles, int per_xcd, const int* __restrict__ rowptr,
     const int* __restrict__ colidx, const T* __restrict__ val, const T* __restrict__ Q, int ldq,
     const T* __restrict__ X1, int ld1, const T* __restrict__ P, int ldp, const T* X2, int ld2,
     T* out, int ldo, T cq, T a1, T b1, T s2, const T* __restrict__ pcq,
-    const T* __restrict__ pa1, const T* __restrict__ pb1, const T* __restrict__ ps2) {
-  constexpr int RPW = 64 / LPR;
-  constexpr int RPB = 4 * RPW;
-  const int tile = xcd_tile(per_xcd);
+    const T* __restrict__ pa1, const T* __restrict__ pb1, const T* __restrict__ ps2,
+    long long chunk_stride) {
+  constexpr int RPW = 64 / LPR;           // teams per wavefront
+  constexpr int RPB = 4 * RPW * RT;       // rows per workgroup tile
+  const int tile = per_xcd ? xcd_tile(per_xcd) : (int)blockIdx.x;
   if (tile >= ntiles) return;
+  const int chunk = tile / nrowtiles;
+  const int rtile = tile - chunk * nrowtiles;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int row = tile * RPB + wave * RPW + lane / LPR;
-  if (row >= n) return;
-  const int tl = lane % LPR;
+  const int row0 = rtile * RPB + (wave * RPW + lane / LPR) * RT;  // RT consecutive rows
+  if (row0 >= n) return;
+  const int cic = (lane % LPR) * V;           // column inside the chunk
+  const int c = chunk * (LPR * V) + cic;      // global column
+  if (c >= b) return;
+  // Row-major blocks: chunk_stride == 0 and the chunk is a column offset.
+  // Panel blocks (dense n x LPR*V slabs): chunk_stride = n*LPR*V, ld = LPR*V.
+  const long long coff = chunk_stride ? (long long)chunk * chunk_stride + cic : c;
+  if (Q) Q += coff;
+  if (X1) X1 += coff;
+  if (P) P += coff;
+  if (X2) X2 += coff;
+  out += coff;
   const bool gather = (X1 != nullptr) && (P == nullptr);
-  int p0 = 0, p1 = 0;
-  if (gather) {
-    p0 = rowptr[row];
-    p1 = rowptr[row + 1];
+  const bool full = (c + V <= b);
+  int p0[RT], len[RT];
+  int maxlen = 0;
+#pragma unroll
+  for (int r = 0; r < RT; ++r) {
+    const int row = min(row0 + r, n - 1);
+    p0[r] = gather ? rowptr[row] : 0;
+    len[r] = gather ? (row0 + r < n ? rowptr[row + 1] - p0[r] : 0) : 0;
+    maxlen = max(maxlen, len[r]);
   }
-  for (int c = tl * V; c < b; c += LPR * V) {
-    T acc[V];
+  T acc[RT][V];
 #pragma unroll
-    for (int e = 0; e < V; ++e) acc[e] = T(0);
-    const bool full = (c + V <= b);
-    if (full) {
-      if (gather) {
-        int p = p0;
-        for (; p + 4 <= p1; p += 4) {
-          const int j0 = colidx[p], j1 = colidx[p + 1], j2 = colidx[p + 2], j3 = colidx[p + 3];
-          const T v0 = val[p], v1 = val[p + 1], v2 = val[p + 2], v3 = val[p + 3];
-          T x0[V], x1[V], x2[V], x3[V];
-          ldv<T, V>(X1 + (size_t)j0 * ld1 + c, x0);
-          ldv<T, V>(X1 + (size_t)j1 * ld1 + c, x1);
-          ldv<T, V>(X1 + (size_t)j2 * ld1 + c, x2);
-          ldv<T, V>(X1 + (size_t)j3 * ld1 + c, x3);
+  for (int r = 0; r < RT; ++r)
 #pragma unroll
-          for (int e = 0; e < V; ++e) {
-            acc[e] += v0 * x0[e];
-            acc[e] += v1 * x1[e];
-            acc[e] += v2 * x2[e];
-            acc[e] += v3 * x3[e];
+    for (int e = 0; e < V; ++e) acc[r][e] = T(0);
+  if (full) {
+    if (gather) {
+      // all index, value and gather loads unconditional (clamped index, zero
+      // weight past the row end) so the RT*U chains of a lane overlap
+      constexpr int U = RT >= 4 ? 1 : (RT == 2 ? 2 : 4);
+      for (int j = 0; j < maxlen; j += U) {
+        int col[U][RT];
+        T w[U][RT];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int r = 0; r < RT; ++r) {
+            const int jj = j + u;
+            const int q = len[r] > 0 ? p0[r] + min(jj, len[r] - 1) : 0;
+            col[u][r] = len[r] > 0 ? colidx[q] : 0;
+            w[u][r] = (jj < len[r]) ? val[q] : T(0);
           }
-        }
-        for (; p < p1; ++p) {
-          const int j0 = colidx[p];
-          const T v0 = val[p];
-          T x0[V];
-          ldv<T, V>(X1 + (size_t)j0 * ld1 + c, x0);
 #pragma unroll
-          for (int e = 0; e < V; ++e) acc[e] += v0 * x0[e];
-        }
-      } else if (P != nullptr) {
-        ldv<T, V>(P + (size_t)row * ldp + c, acc);
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int r = 0; r < RT; ++r) {
+            T x[V];
+            ldv<T, V>(X1 + (size_t)col[u][r] * ld1, x);
+#pragma unroll
+            for (int e = 0; e < V; ++e) acc[r][e] += w[u][r] * x[e];
+          }
       }
-      T r[V];
+    }
 #pragma unroll
-      for (int e = 0; e < V; ++e) r[e] = (PERCOL ? pa1[c + e] : a1) * acc[e];
+    for (int r = 0; r < RT; ++r) {
+      const int row = row0 + r;
+      if (row >= n) break;
+      if (P != nullptr) ldv<T, V>(P + (size_t)row * ldp, acc[r]);
+      T res[V];
+#pragma unroll
+      for (int e = 0; e < V; ++e) res[e] = (PERCOL ? pa1[c + e] : a1) * acc[r][e];
       if (Q != nullptr) {
         T q[V];
-        ldv<T, V>(Q + (size_t)row * ldq + c, q);
+        ldv<T, V>(Q + (size_t)row * ldq, q);
 #pragma unroll
-        for (int e = 0; e < V; ++e) r[e] = (PERCOL ? pcq[c + e] : cq) * q[e] + r[e];
+        for (int e = 0; e < V; ++e) res[e] = (PERCOL ? pcq[c + e] : cq) * q[e] + res[e];
       }
       if (X1 != nullptr) {
         T x[V];
-        ldv<T, V>(X1 + (size_t)row * ld1 + c, x);
+        ldv<T, V>(X1 + (size_t)row * ld1, x);
 #pragma unroll
-        for (int e = 0; e < V; ++e) r[e] += (PERCOL ? pb1[c + e] : b1) * x[e];
+        for (int e = 0; e < V; ++e) res[e] += (PERCOL ? pb1[c + e] : b1) * x[e];
       }
       if (X2 != nullptr) {
         T y[V];
-        ldv_alias<T, V>(X2 + (size_t)row * ld2 + c, y);
+        ldv_alias<T, V>(X2 + (size_t)row * ld2, y);
 #pragma unroll
-        for (int e = 0; e < V; ++e) r[e] -= (PERCOL ? ps2[c + e] : s2) * y[e];
+        for (int e = 0; e < V; ++e) res[e] -= (PERCOL ? ps2[c + e] : s2) * y[e];
       }
-      stv<T, V>(out + (size_t)row * ldo + c, r);
-    } else {
-      // ragged tail of the row: element-wise
+      stv<T, V>(out + (size_t)row * ldo, res);
+    }
+  } else {
+    // ragged tail of the block width: element-wise
+    for (int r = 0; r < RT; ++r) {
+      const int row = row0 + r;
+      if (row >= n) break;
       for (int e = 0; e < V && c + e < b; ++e) {
-        const int cc = c + e;
+        const int cc = c + e;  // global column (per-column scalars)
         T a = T(0);
         if (gather) {
-          for (int p = p0; p < p1; ++p) a += val[p] * X1[(size_t)colidx[p] * ld1 + cc];
+          for (int p = p0[r]; p < p0[r] + len[r]; ++p) a += val[p] * X1[(size_t)colidx[p] * ld1 + e];
         } else if (P != nullptr) {
-          a = P[(size_t)row * ldp + cc];
+          a = P[(size_t)row * ldp + e];
         }
-        T r = (PERCOL ? pa1[cc] : a1) * a;
-        if (Q != nullptr) r = (PERCOL ? pcq[cc] : cq) * Q[(size_t)row * ldq + cc] + r;
-        if (X1 != nullptr) r += (PERCOL ? pb1[cc] : b1) * X1[(size_t)row * ld1 + cc];
-        if (X2 != nullptr) r -= (PERCOL ? ps2[cc] : s2) * X2[(size_t)row * ld2 + cc];
-        out[(size_t)row * ldo + cc] = r;
+        T res = (PERCOL ? pa1[cc] : a1) * a;
+        if (Q != nullptr) res = (PERCOL ? pcq[cc] : cq) * Q[(size_t)row * ldq + e] + res;
+        if (X1 != nullptr) res += (PERCOL ? pb1[cc] : b1) * X1[(size_t)row * ld1 + e];
+        if (X2 != nullptr) res -= (PERCOL ? ps2[cc] : s2) * X2[(size_t)row * ld2 + e];
+        out[(size_t)row * ldo + e] = res;
       }
     }
   }
 }
 
-template <typename T, int V, int LPR, bool PERCOL>
+template <typename T, int V, int LPR, int RT, bool PERCOL>
 static void clenshaw_launch_t(hipStream_t st, const DevCSR& A, int b, const T* Q, int ldq,
                               const T* X1, int ld1, const T* P, int ldp, const T* X2, int ld2,
                               T* out, int ldo, T cq, T a1, T b1, T s2, const T* pcq, const T* pa1,
-                              const T* pb1, const T* ps2) {
-  constexpr int RPB = 4 * (64 / LPR);
-  const int ntiles = (A.n + RPB - 1) / RPB;
+                              const T* pb1, const T* ps2, bool panels) {
+  constexpr int RPB = 4 * (64 / LPR) * RT;
+  const int nrowtiles = (A.n + RPB - 1) / RPB;
+  const int nchunks = (b + LPR * V - 1) / (LPR * V);
+  const long long nt = (long long)nrowtiles * nchunks;
+  if (nt > 0x7fffff00LL) throw Error(-3, "filter grid too large");
   int per_xcd = 0;
-  const int grid = grid_xcd(ntiles, &per_xcd);
-  hipLaunchKernelGGL((clenshaw_kernel<T, V, LPR, PERCOL>), dim3(grid), dim3(256), 0, st, A.n, b,
-                     ntiles, per_xcd, A.rowptr, A.col, A.vals<T>(), Q, ldq, X1, ld1, P, ldp, X2,
-                     ld2, out, ldo, cq, a1, b1, s2, pcq, pa1, pb1, ps2);
+  int grid = grid_xcd((int)nt, &per_xcd);
+  if (!xcd_remap_enabled()) {
+    per_xcd = 0;
+    grid = (int)nt;
+  }
+  hipLaunchKernelGGL((clenshaw_kernel<T, V, LPR, RT, PERCOL>), dim3(grid), dim3(256), 0, st, A.n, b,
+                     nrowtiles, (int)nt, per_xcd, A.rowptr, A.col, A.vals<T>(), Q, ldq, X1, ld1, P,
+                     ldp, X2, ld2, out, ldo, cq, a1, b1, s2, pcq, pa1, pb1, ps2,
+                     panels ? (long long)A.n * (LPR * V) : 0LL);
   SMALGO_HIP(hipGetLastError());
 }
 
@@ -196,18 +242,23 @@ template <typename T, bool PERCOL>
 static void clenshaw_dispatch(hipStream_t st, const DevCSR& A, int b, const T* Q, int ldq,
                               const T* X1, int ld1, const T* P, int ldp, const T* X2, int ld2,
                               T* out, int ldo, T cq, T a1, T b1, T s2, const T* pcq, const T* pa1,
-                              const T* pb1, const T* ps2, FilterTile t) {
-#define SMALGO_CL(VV, LL)                                                                    \
-  if (t.vec == VV && t.lpr == LL)                                                            \
-    return clenshaw_launch_t<T, VV, LL, PERCOL>(st, A, b, Q, ldq, X1, ld1, P, ldp, X2, ld2,  \
-                                                out, ldo, cq, a1, b1, s2, pcq, pa1, pb1, ps2);
-#define SMALGO_CL_V(VV) \
-  SMALGO_CL(VV, 2) SMALGO_CL(VV, 4) SMALGO_CL(VV, 8) SMALGO_CL(VV, 16) SMALGO_CL(VV, 32) SMALGO_CL(VV, 64)
-  SMALGO_CL_V(1) SMALGO_CL_V(2) SMALGO_CL_V(4)
+                              const T* pb1, const T* ps2, FilterTile t, bool panels = false) {
+#define SMALGO_CL(VV, LL, RR)                                                                  \
+  if (t.vec == VV && t.lpr == LL && t.rt == RR)                                                \
+    return clenshaw_launch_t<T, VV, LL, RR, PERCOL>(st, A, b, Q, ldq, X1, ld1, P, ldp, X2, ld2, \
+                                                    out, ldo, cq, a1, b1, s2, pcq, pa1, pb1, ps2, \
+                                                    panels);
+#define SMALGO_CL_V(VV, RR) \
+  SMALGO_CL(VV, 4, RR) SMALGO_CL(VV, 8, RR) SMALGO_CL(VV, 16, RR) SMALGO_CL(VV, 32, RR) SMALGO_CL(VV, 64, RR)
+  SMALGO_CL_V(1, 1) SMALGO_CL_V(2, 1) SMALGO_CL_V(4, 1)
+  if constexpr (!PERCOL) {
+    SMALGO_CL_V(1, 2) SMALGO_CL_V(2, 2) SMALGO_CL_V(4, 2)
+    SMALGO_CL_V(1, 4) SMALGO_CL_V(2, 4) SMALGO_CL_V(4, 4)
+  }
 #undef SMALGO_CL_V
 #undef SMALGO_CL
   throw Error(-3, "unsupported filter tile vec=" + std::to_string(t.vec) +
-                      " lpr=" + std::to_string(t.lpr));
+                      " lpr=" + std::to_string(t.lpr) + " rt=" + std::to_string(t.rt));
 }
 
 static void check_tile_alignment(FilterTile t, size_t elem, std::initializer_list<const void*> ptrs,
@@ -241,6 +292,52 @@ void launch_clenshaw_step_cols(hipStream_t st, const DevCSR& A, int b, const T* 
                              T(0), T(0), cq, a1, b1, s2, tile);
 }
 
+// pack a row-major block (ld) into panels of width pw; unpack the reverse
+template <typename T>
+__global__ void to_panels_kernel(int n, int b, int pw, const T* __restrict__ S, int lds,
+                                 T* __restrict__ D) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int npan = (b + pw - 1) / pw;
+  if (idx >= (size_t)n * npan * pw) return;
+  const int cc = (int)(idx % pw);
+  const size_t r = idx / pw;
+  const int i = (int)(r % n), pan = (int)(r / n);
+  const int c = pan * pw + cc;
+  D[idx] = c < b ? S[(size_t)i * lds + c] : T(0);
+}
+template <typename T>
+__global__ void from_panels_kernel(int n, int b, int pw, const T* __restrict__ S,
+                                   T* __restrict__ D, int ldd) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int npan = (b + pw - 1) / pw;
+  if (idx >= (size_t)n * npan * pw) return;
+  const int cc = (int)(idx % pw);
+  const size_t r = idx / pw;
+  const int i = (int)(r % n), pan = (int)(r / n);
+  const int c = pan * pw + cc;
+  if (c < b) D[(size_t)i * ldd + c] = S[idx];
+}
+template <typename T>
+void to_panels(hipStream_t st, int n, int b, int pw, const T* S, int lds, T* D) {
+  const size_t tot = (size_t)n * ((b + pw - 1) / pw) * pw;
+  if (!tot) return;
+  hipLaunchKernelGGL((to_panels_kernel<T>), dim3((tot + 255) / 256), dim3(256), 0, st, n, b, pw, S,
+                     lds, D);
+  SMALGO_HIP(hipGetLastError());
+}
+template <typename T>
+void from_panels(hipStream_t st, int n, int b, int pw, const T* S, T* D, int ldd) {
+  const size_t tot = (size_t)n * ((b + pw - 1) / pw) * pw;
+  if (!tot) return;
+  hipLaunchKernelGGL((from_panels_kernel<T>), dim3((tot + 255) / 256), dim3(256), 0, st, n, b, pw,
+                     S, D, ldd);
+  SMALGO_HIP(hipGetLastError());
+}
+template void to_panels<float>(hipStream_t, int, int, int, const float*, int, float*);
+template void to_panels<double>(hipStream_t, int, int, int, const double*, int, double*);
+template void from_panels<float>(hipStream_t, int, int, int, const float*, float*, int);
+template void from_panels<double>(hipStream_t, int, int, int, const double*, double*, int);
+
 // Axpby form with a precomputed operator product P = A·X1 (host-callback
 // operators): out = cq*Q + a1*P + b1*X1 - s2*X2.
 void launch_clenshaw_step_pre(hipStream_t st, int n, int b, const double* Q, int ldq, double cq,
@@ -250,7 +347,7 @@ void launch_clenshaw_step_pre(hipStream_t st, int n, int b, const double* Q, int
   if (n == 0 || b == 0) return;
   DevCSR fake;
   fake.n = n;
-  FilterTile t{1, 64};
+  FilterTile t{1, 64, 1};
   clenshaw_dispatch<double, false>(st, fake, b, Q, ldq, X1, ld1, P, ldp, X2, ld2, out, ldo, cq, a1,
                                    b1, s2, nullptr, nullptr, nullptr, nullptr, t);
 }
@@ -502,7 +599,8 @@ SMALGO_GEMM_INST(float, double, float)
 static void colred_shape(int n, int b, int* nblk, int* rows_per_blk, int* nrt) {
   const int nct = std::min(b, 256);
   *nrt = std::max(1, 256 / std::max(nct, 1));
-  int want = 4 * device_cu_count();
+  // at most ~2048 partial sums per column
+  int want = std::max(1, std::min(4 * device_cu_count(), 2048 / *nrt));
   int rpb = std::max(64, (n + want - 1) / want);
   *rows_per_blk = rpb;
   *nblk = std::max(1, (n + rpb - 1) / rpb);
@@ -544,13 +642,24 @@ __global__ __launch_bounds__(256) void colred_kernel(int n, int b, int rows_per_
   }
 }
 
+// one workgroup per group of 16 columns: 16 threads per column stride over the
+// partial sums, then a fixed-order tree in LDS (deterministic)
 template <int MODE>
-__global__ void colred_finish(int b, int nparts, const double* __restrict__ part, double* out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= b) return;
+__global__ __launch_bounds__(256) void colred_finish(int b, int nparts,
+                                                     const double* __restrict__ part, double* out) {
+  __shared__ double red[16][17];
+  const int cl = threadIdx.x & 15, pl = threadIdx.x >> 4;
+  const int c = blockIdx.x * 16 + cl;
   double s = 0.0;
-  for (int p = 0; p < nparts; ++p) s += part[(size_t)p * b + c];
-  out[c] = MODE == 1 ? sqrt(s) : s;
+  if (c < b)
+    for (int p = pl; p < nparts; p += 16) s += part[(size_t)p * b + c];
+  red[pl][cl] = s;
+  __syncthreads();
+  for (int h = 8; h > 0; h >>= 1) {
+    if (pl < h) red[pl][cl] += red[pl + h][cl];
+    __syncthreads();
+  }
+  if (pl == 0 && c < b) out[c] = MODE == 1 ? sqrt(red[0][cl]) : red[0][cl];
 }
 
 template <typename TX, int MODE>
@@ -563,7 +672,7 @@ static void colred(hipStream_t st, int n, int b, const TX* X, int ldx, const TX*
   hipLaunchKernelGGL((colred_kernel<TX, MODE>), dim3(nblk), dim3(256), 0, st, n, b, rpb, nrt, X,
                      ldx, Y, ldy, lam, work);
   SMALGO_HIP(hipGetLastError());
-  hipLaunchKernelGGL((colred_finish<MODE>), dim3((b + 255) / 256), dim3(256), 0, st, b, nblk * nrt,
+  hipLaunchKernelGGL((colred_finish<MODE>), dim3((b + 15) / 16), dim3(256), 0, st, b, nblk * nrt,
                      work, out);
   SMALGO_HIP(hipGetLastError());
 }
@@ -807,6 +916,22 @@ void zero_lower_cm(hipStream_t st, int b, double* C, int ldc) {
   if (b <= 1) return;
   dim3 blk(16, 16), grid((b + 15) / 16, (b + 15) / 16);
   hipLaunchKernelGGL(zero_lower_cm_kernel, grid, blk, 0, st, b, C, ldc);
+  SMALGO_HIP(hipGetLastError());
+}
+
+// plain streaming copy (16-byte vectors, grid-stride) for bandwidth calibration
+__global__ __launch_bounds__(256) void stream_copy_kernel(const float4* __restrict__ a,
+                                                          float4* __restrict__ b, size_t n4) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+void stream_copy(hipStream_t st, const void* src, void* dst, size_t bytes) {
+  const size_t n4 = bytes / 16;
+  if (!n4) return;
+  const int grid = 8 * device_cu_count();
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(grid), dim3(256), 0, st,
+                     reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n4);
   SMALGO_HIP(hipGetLastError());
 }
 

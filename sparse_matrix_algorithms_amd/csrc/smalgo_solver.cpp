@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <limits>
 #include <numeric>
 
@@ -103,10 +104,13 @@ FilterTile choose_tile(size_t elem, int b, std::initializer_list<const void*> pt
     return true;
   };
   while (vec > 1 && (!ok(vec) || b < vec)) vec /= 2;
-  int need = (b + vec - 1) / vec;
-  int lpr = 2;
+  // one column chunk of at most 512 bytes per row: keeps the rows that a
+  // row's CSR neighbours gather inside the per-XCD L2 for wide blocks
+  const int chunk_elems = std::max(vec, (int)(512 / elem));
+  int need = (std::min(b, chunk_elems) + vec - 1) / vec;
+  int lpr = 4;
   while (lpr < need && lpr < 64) lpr *= 2;
-  return FilterTile{vec, lpr};
+  return FilterTile{vec, lpr, 1};
 }
 
 // ---------------------------------------------------------------------------
@@ -199,10 +203,70 @@ static void filter_step(Ctx& cx, const Op& op, int b, const T* Q, int ldq, doubl
                           (T)s2e, out, ldo, t);
 }
 
+// Autotuner: on the first filter of a (precision, width, matrix, alignment)
+// combination, time a few fused steps of every admissible (vec, lanes per
+// row) tile on the real operands and keep the fastest.  Rows per wavefront is
+// 64/lpr; the column chunk per row is lpr*vec elements.
 template <typename T>
-void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::vector<double>& co,
-                  double smin, double smax, T* Y, int ldy, T* buf0, T* buf1, int ldb,
-                  double* pbuf) {
+static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ldq, T* buf0, T* buf1,
+                                int ldb, T* Y, int ldy) {
+  const FilterTile def = choose_tile(sizeof(T), b, {Q, buf0, buf1, Y}, {ldq, ldb, ldy});
+  if (!cx.autotune) return def;
+  auto al = [&](int v) {
+    for (const void* p : {(const void*)Q, (const void*)buf0, (const void*)buf1, (const void*)Y})
+      if (reinterpret_cast<uintptr_t>(p) % (sizeof(T) * v)) return 0;
+    for (int ld : {ldq, ldb, ldy})
+      if (ld % v) return 0;
+    return 1;
+  };
+  const long long key = ((long long)sizeof(T) << 56) ^ ((long long)b << 32) ^
+                        ((long long)op.mat_id << 4) ^ (al(4) << 2) ^ (al(2) << 1);
+  auto it = cx.tuned.find(key);
+  if (it != cx.tuned.end()) return it->second;
+  std::vector<FilterTile> cand;
+  for (int v : {1, 2, 4}) {
+    if (!al(v) || (v > 1 && b < v)) continue;
+    for (int l : {4, 8, 16, 32, 64}) {
+      const int w = l * v;  // chunk width in elements
+      if (w > 2 * std::max(b, 4) || w * (int)sizeof(T) > 2048) continue;
+      for (int rt : {1, 2, 4}) cand.push_back(FilterTile{v, l, rt});
+    }
+  }
+  if (cand.empty()) return def;
+  hipEvent_t e0, e1;
+  SMALGO_HIP(hipEventCreate(&e0));
+  SMALGO_HIP(hipEventCreate(&e1));
+  FilterTile best = def;
+  float best_ms = 1e30f;
+  for (const FilterTile& t : cand) {
+    // warm, then time 4 steps of the general form out = cq Q + a1 A X1 + b1 X1 - X2
+    launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), Q, ldq, T(0.1), T(0.1), buf1,
+                            ldb, T(1), buf0, ldb, t);
+    SMALGO_HIP(hipEventRecord(e0, cx.stream));
+    for (int r = 0; r < 4; ++r)
+      launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), Q, ldq, T(0.1), T(0.1), buf1,
+                              ldb, T(1), buf0, ldb, t);
+    SMALGO_HIP(hipEventRecord(e1, cx.stream));
+    SMALGO_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  cx.tuned[key] = best;
+  return best;
+}
+
+// The recurrence itself, written once over a "step" callback that launches
+// out = cq*Q + a1*(A X1) + b1*X1 - s2*X2 with X1/X2 given as (pointer,
+// implicit multiple of Q).
+template <typename T, typename StepFn>
+static void clenshaw_recurrence(const std::vector<double>& co, double smin, double smax, T* buf0,
+                                T* buf1, T* Y, StepFn&& step) {
   // Block Clenshaw exactly as apply_poly_block! (polfed.jl:464-501):
   //   B_{K+1} = B_{K+2} = 0
   //   for k = K+1 .. 2:  TMP = c[k-1] Q + α A B1 + β B1 - B2;  (B1,B2) <- (TMP,B1)
@@ -217,14 +281,6 @@ void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::
   const double al = 2.0 * invd, be = -2.0 * c * invd;
   FBlk<T> B1{nullptr, 0.0}, B2{nullptr, 0.0};
   T* bufs[2] = {buf0, buf1};
-  FilterTile tile{};
-  const FilterTile* tp = nullptr;
-  if (!op.is_callback()) {
-    tile = choose_tile(sizeof(T), b, {Q, buf0, buf1, Y}, {ldq, ldb, ldy});
-    auto it = cx.tuned.find(((long long)sizeof(T) << 40) ^ ((long long)b << 20) ^ op.mat_id);
-    if (it != cx.tuned.end()) tile = it->second;
-    tp = &tile;
-  }
   for (int k = K + 1; k >= 2; --k) {
     const double ck = co[k - 1];
     if (k == K + 1) {
@@ -233,11 +289,96 @@ void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::
       continue;
     }
     T* out = B2.p ? const_cast<T*>(B2.p) : (B1.p == bufs[0] ? bufs[1] : bufs[0]);
-    filter_step<T>(cx, op, b, Q, ldq, ck, al, be, B1, B2, ldb, out, ldb, pbuf, tp);
+    step(ck, al, be, B1, B2, out);
     B2 = B1;
     B1 = {out, 0.0};
   }
-  filter_step<T>(cx, op, b, Q, ldq, co[0], invd, -c * invd, B1, B2, ldb, Y, ldy, pbuf, tp);
+  step(co[0], invd, -c * invd, B1, B2, Y);
+}
+
+// panel width for wide blocks: 512-byte row segments per slab
+template <typename T>
+static constexpr int panel_width() { return 512 / (int)sizeof(T); }
+
+// Tuning overrides (environment): SMALGO_FILTER_TILE="vec,lpr,rt" fixes the
+// tile of row-major filters, SMALGO_PANEL_TILE likewise for panel filters
+// (lpr*vec must equal the panel width), SMALGO_NO_PANELS=1 keeps wide blocks
+// row-major.
+static bool env_tile(const char* name, FilterTile* t) {
+  const char* v = std::getenv(name);
+  if (!v) return false;
+  int a = 0, b = 0, c = 1;
+  if (std::sscanf(v, "%d,%d,%d", &a, &b, &c) < 2) return false;
+  *t = FilterTile{a, b, c};
+  return true;
+}
+
+template <typename T>
+void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::vector<double>& co,
+                  double smin, double smax, T* Y, int ldy, T* buf0, T* buf1, int ldb,
+                  double* pbuf) {
+  const char* nop = std::getenv("SMALGO_NO_PANELS");
+  const bool panels_ok = !(nop && nop[0] == '1');
+  if (!op.is_callback() && panels_ok && b > panel_width<T>()) {
+    // Wide blocks: repack into dense n x PW slabs so that every gather and
+    // stream of one launch chunk walks contiguous rows (DRAM page locality),
+    // then unpack the result.
+    const int pw = panel_width<T>();
+    const int npan = (b + pw - 1) / pw;
+    const int n = op.n;
+    const size_t sz = (size_t)n * npan * pw;
+    DBuf<T> Qp(sz), P0(sz), P1(sz), Yp(sz);
+    to_panels<T>(cx.stream, n, b, pw, Q, ldq, Qp.p);
+    // one launch per panel and step: each launch has the shape of the
+    // narrow-block filter (the tile the autotuner picks for width pw), which
+    // runs faster per byte than one launch spanning all panels
+    const size_t ps = (size_t)n * pw;
+    FilterTile tile{};
+    if (!env_tile("SMALGO_PANEL_TILE", &tile))
+      tile = autotune_tile<T>(cx, op, pw, Qp.p, pw, P0.p, P1.p, pw, Yp.p, pw);
+    auto step = [&](double cq, double a1, double b1, FBlk<T> X1, FBlk<T> X2, T* out) {
+      const T* x1p = X1.p;
+      double a1e = a1, b1e = b1;
+      if (!x1p) {
+        if (X1.qs == 0.0) {
+          a1e = b1e = 0.0;
+        } else {
+          x1p = Qp.p;
+          a1e = a1 * X1.qs;
+          b1e = b1 * X1.qs;
+        }
+      }
+      const T* x2p = X2.p;
+      double cqe = cq, s2e = 1.0;
+      if (!x2p) {
+        cqe = cq - X2.qs;
+        s2e = 0.0;
+      }
+      for (int pn = 0; pn < npan; ++pn) {
+        const int bp = std::min(pw, b - pn * pw);
+        const size_t o = (size_t)pn * ps;
+        launch_clenshaw_step<T>(cx.stream, *op.A, bp, Qp.p + o, pw, (T)cqe, x1p ? x1p + o : nullptr,
+                                pw, (T)a1e, (T)b1e, x2p ? x2p + o : nullptr, pw, (T)s2e, out + o,
+                                pw, tile);
+      }
+    };
+    clenshaw_recurrence<T>(co, smin, smax, P0.p, P1.p, Yp.p, step);
+    from_panels<T>(cx.stream, n, b, pw, Yp.p, Y, ldy);
+    return;
+  }
+  FilterTile tile{};
+  const FilterTile* tp = nullptr;
+  if (!op.is_callback()) {
+    if (!env_tile("SMALGO_FILTER_TILE", &tile))
+      tile = autotune_tile<T>(cx, op, b, Q, ldq, buf0, buf1, ldb, Y, ldy);
+    tp = &tile;
+    cx.last_tile = tile;
+  }
+  auto step = [&](double cq, double a1, double b1, FBlk<T> X1, FBlk<T> X2, T* out) {
+    const int ldo = (out == Y) ? ldy : ldb;
+    filter_step<T>(cx, op, b, Q, ldq, cq, a1, b1, X1, X2, ldb, out, ldo, pbuf, tp);
+  };
+  clenshaw_recurrence<T>(co, smin, smax, buf0, buf1, Y, step);
 }
 template void apply_filter<double>(Ctx&, const Op&, int, const double*, int,
                                    const std::vector<double>&, double, double, double*, int,
@@ -683,6 +824,7 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
   DBuf<double> gw(gwn);
   std::vector<double> hA((size_t)s * s), R;
   int r_prev = 0;
+  FilterTile basis_tile{};
   double t_filter = 0;
   long long filter_steps = 0;
   double gemm_flops = 0;
@@ -693,6 +835,7 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
     size_t hf = tm.begin("filter");
     apply_filter<T>(cx, op, r, Qj, ldV, coeffs, smin, smax, W.p, lds, b0.p, b1.p, lds, pbuf.p);
     tm.end(hf);
+    if (j == 0) basis_tile = cx.last_tile;
     filter_steps += degree;  // K fused steps (the all-zero first step is skipped)
     size_t ho = tm.begin("orth");
     // W -= Q_{j-1} B_{j-1}ᵀ   (polfed.jl:1073-1075)
@@ -1039,6 +1182,9 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
   out->stats["gemm_flops"] = gemm_flops;
   out->stats["smin"] = smin;
   out->stats["smax"] = smax;
+  out->stats["tile_vec"] = basis_tile.vec;
+  out->stats["tile_lpr"] = basis_tile.lpr;
+  out->stats["tile_rt"] = basis_tile.rt;
   // algorithmic bytes of the basis-build filter: per fused step
   // nnz*(w+4) + 4(n+1) + 4*n*b*w   (b = block width, w = sizeof(T))
   double bytes = 0;

@@ -45,7 +45,9 @@ struct Ctx {
   hipStream_t stream = nullptr;
   rocblas_handle blas = nullptr;
   std::string err;
-  std::map<long long, FilterTile> tuned;  // (precision, b, matrix id) -> tile
+  std::map<long long, FilterTile> tuned;  // (precision, b, matrix id, alignment) -> tile
+  bool autotune = true;
+  FilterTile last_tile{};  // tile of the most recent row-major filter
 };
 
 struct Mat {

@@ -1,0 +1,58 @@
+"""Filter-kernel profiling driver (run under rocprofv3).
+
+Runs the fused Clenshaw filter on the benchmark workload (3D Anderson,
+W=16.5, block width b) for a fixed degree through the C ABI, plus a plain
+copy kernel of known size (bytes read = bytes written = ``--copy-mib`` MiB)
+so that FETCH_SIZE / WRITE_SIZE counters can be scaled to bytes.
+
+    python -m sparse_matrix_algorithms_amd.profile_filter --L 48 --b 64 --prec fp32 --degree 50
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import sys
+import time
+
+import numpy as np
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--L", type=int, default=48)
+    ap.add_argument("--W", type=float, default=16.5)
+    ap.add_argument("--seed", type=int, default=48)
+    ap.add_argument("--b", type=int, default=64)
+    ap.add_argument("--prec", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--degree", type=int, default=50)
+    ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--copy-mib", type=int, default=256)
+    args = ap.parse_args()
+
+    import sparse_matrix_algorithms_amd as sm
+    from sparse_matrix_algorithms_amd.lattices import anderson3d, gaussian_block
+
+    A = anderson3d(args.L, args.W, args.seed)
+    n = A.shape[0]
+    ctx = sm.default_context()
+    dm = sm.DeviceMatrix(A, ctx)
+    smin, smax = dm.gershgorin_bounds()
+    co = sm.build_coeffs(args.degree, smin, smax, 0.0, -0.04, 0.04, "step")
+    Q = gaussian_block(1, n, args.b)
+    for r in range(args.reps):
+        t = time.perf_counter()
+        sm.apply_poly_block(dm, Q, co, smin, smax, precision=args.prec, ctx=ctx)
+        print(f"filter rep {r}: {time.perf_counter() - t:.4f}s (degree {args.degree}, "
+              f"n={n}, b={args.b}, {args.prec})", flush=True)
+    w = 4 if args.prec == "fp32" else 8
+    per_launch = dm.nnz * (w + 4) + 4 * (n + 1) + 4 * n * args.b * w
+    print(f"algorithmic bytes per fused step: {per_launch}", flush=True)
+    sec = C.c_double()
+    nbytes = args.copy_mib << 20
+    ctx.check(ctx.lib.smalgo_copy_bench(ctx.h, nbytes, 20, C.byref(sec)))
+    print(f"copy kernel: {nbytes} B read + {nbytes} B written per launch, {sec.value * 1e6:.1f} us "
+          f"-> {2 * nbytes / sec.value / 1e9:.1f} GB/s", flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

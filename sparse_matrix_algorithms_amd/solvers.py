@@ -254,7 +254,8 @@ def apply_poly_block(A, Q: np.ndarray, coeffs: np.ndarray, smin: float, smax: fl
 # ---------------------------------------------------------------------------
 _STATS = ("t_total", "t_filter", "t_orth", "t_extract", "t_refine", "filter_steps",
           "block_steps", "basis_dim", "iterations", "spmm_bytes", "gemm_flops",
-          "refine_filter_steps", "locked", "smin", "smax", "warn_interval")
+          "refine_filter_steps", "locked", "smin", "smax", "warn_interval", "tile_vec",
+          "tile_lpr", "tile_rt")
 
 
 def _take_result(ctx: Context, h, want_X: bool = True):

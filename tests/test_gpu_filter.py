@@ -40,7 +40,7 @@ def test_filter_fp64_degrees(ctx, K, window):
     assert err <= 1e-12, err
 
 
-@pytest.mark.parametrize("b", [4, 16, 64, 128])
+@pytest.mark.parametrize("b", [4, 16, 64, 128, 300])
 def test_filter_fp32(ctx, b):
     """FP32 filter: relative error within 1e-4 of the FP64 oracle (K=100)."""
     import sparse_matrix_algorithms_amd as sm
