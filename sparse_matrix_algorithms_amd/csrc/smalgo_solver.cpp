@@ -1024,17 +1024,31 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
           int ldyr = 4;
           const int kq = refine_rr(cx, dn, op, n, kz, Ytmp.p, ldz, rr, Yr, ldyr, lam_act, res_act);
           if (kq == 0) break;
-          // lock converged inside pairs
-          std::vector<int> keep, lock;
+          // lock converged inside pairs; keep the unconverged inside pairs and
+          // only the guard vectors nearest the interval (at most half as many
+          // as unconverged pairs, at least 16) for the next iteration
+          std::vector<int> keep, lock, outside;
           int inside_unconv = 0;
           for (int i = 0; i < kq; ++i) {
             const bool inside = p.emin <= lam_act[i] && lam_act[i] <= p.emax;
             if (inside && res_act[i] <= tol_abs && nlock + (int)lock.size() < ldz)
               lock.push_back(i);
-            else {
+            else if (inside) {
               keep.push_back(i);
-              if (inside) ++inside_unconv;
+              ++inside_unconv;
+            } else {
+              outside.push_back(i);
             }
+          }
+          {
+            auto dist = [&](int i) {
+              return lam_act[i] < p.emin ? p.emin - lam_act[i] : lam_act[i] - p.emax;
+            };
+            std::stable_sort(outside.begin(), outside.end(),
+                             [&](int a, int b) { return dist(a) < dist(b); });
+            const int ng = std::min((int)outside.size(), std::max(16, inside_unconv / 2));
+            keep.insert(keep.end(), outside.begin(), outside.begin() + ng);
+            std::sort(keep.begin(), keep.end());
           }
           if (!lock.empty()) {
             DBuf<int> dl = upload_idx(cx, lock);
