@@ -155,6 +155,25 @@ int smalgo_kpm_moments(smalgo_ctx* ctx, const smalgo_mat* A, int nmom, int R, ui
  * copy; used to calibrate HBM byte counters (FETCH_SIZE / WRITE_SIZE). */
 int smalgo_copy_bench(smalgo_ctx* ctx, int64_t bytes, int reps, double* seconds);
 
+/* ---- learned filter parameters (no reference counterpart) -----------------
+ * FP32 two-layer perceptron nin -> nh (ReLU) -> nout with hand-written HIP
+ * forward, backward (mean squared error) and Adam kernels.  Parameters are
+ * one FP32 vector [W1 (nin x nh, row-major), b1 (nh), W2 (nh x nout), b2 (nout)].
+ * X: B x nin row-major, T/Y: B x nout row-major (host FP32 arrays). */
+typedef struct smalgo_mlp smalgo_mlp;
+int smalgo_mlp_create(smalgo_ctx* ctx, int nin, int nh, int nout, uint64_t seed, smalgo_mlp** out);
+void smalgo_mlp_free(smalgo_mlp* m);
+int64_t smalgo_mlp_num_params(const smalgo_mlp* m);
+int smalgo_mlp_get_params(const smalgo_mlp* m, float* params);
+int smalgo_mlp_set_params(smalgo_mlp* m, const float* params);
+int smalgo_mlp_predict(smalgo_mlp* m, const float* X, int B, float* Y);
+/* loss = mean((Y - T)^2); grad (optional, num_params floats) = dloss/dparams */
+int smalgo_mlp_gradient(smalgo_mlp* m, const float* X, const float* T, int B, float* grad,
+                        float* loss);
+/* `epochs` full-batch Adam steps (β1=0.9, β2=0.999, ε=1e-8); loss_hist optional */
+int smalgo_mlp_train(smalgo_mlp* m, const float* X, const float* T, int B, int epochs, float lr,
+                     float* loss_hist, float* loss);
+
 /* ---- POLFED ---------------------------------------------------------------
  * Keyword arguments of polfed(A::MatrixFreeRealSymOp,nev,degree;...)
  * (polfed.jl:1128) plus the mixed-precision controls.  has_* flags stand in

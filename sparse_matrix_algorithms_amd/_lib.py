@@ -50,6 +50,7 @@ class ChebParams(C.Structure):
 _P = C.c_void_p
 _DP = C.POINTER(C.c_double)
 _I64P = C.POINTER(C.c_int64)
+_FP = C.POINTER(C.c_float)
 SIGNATURES = [
     ("smalgo_ctx_create", C.c_int, [C.c_int, C.POINTER(_P)]),
     ("smalgo_ctx_destroy", None, [_P]),
@@ -90,6 +91,14 @@ SIGNATURES = [
     ("smalgo_kpm_moments", C.c_int,
      [_P, _P, C.c_int, C.c_int, C.c_uint64, C.c_double, C.c_double, _DP]),
     ("smalgo_copy_bench", C.c_int, [_P, C.c_int64, C.c_int, _DP]),
+    ("smalgo_mlp_create", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(_P)]),
+    ("smalgo_mlp_free", None, [_P]),
+    ("smalgo_mlp_num_params", C.c_int64, [_P]),
+    ("smalgo_mlp_get_params", C.c_int, [_P, _FP]),
+    ("smalgo_mlp_set_params", C.c_int, [_P, _FP]),
+    ("smalgo_mlp_predict", C.c_int, [_P, _FP, C.c_int, _FP]),
+    ("smalgo_mlp_gradient", C.c_int, [_P, _FP, _FP, C.c_int, _FP, _FP]),
+    ("smalgo_mlp_train", C.c_int, [_P, _FP, _FP, C.c_int, C.c_int, C.c_float, _FP, _FP]),
     ("smalgo_polfed_default_params", None, [C.POINTER(PolfedParams)]),
     ("smalgo_polfed", C.c_int,
      [_P, _P, MATBLOCK_FN, _P, C.c_int64, C.c_int, C.c_int, C.POINTER(PolfedParams), _DP,

@@ -9,6 +9,7 @@
 
 #include "../../include/smalgo.h"
 #include "smalgo_coeffs.h"
+#include "smalgo_mlp.h"
 #include "smalgo_solver.h"
 
 using namespace smalgo;
@@ -392,6 +393,62 @@ int smalgo_copy_bench(smalgo_ctx* ctx, int64_t bytes, int reps, double* seconds)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *seconds = ms * 1e-3 / reps;
+  });
+}
+
+// ---- MLP -------------------------------------------------------------------
+struct smalgo_mlp {
+  smalgo_ctx* ctx;
+  Mlp* m;
+};
+
+int smalgo_mlp_create(smalgo_ctx* ctx, int nin, int nh, int nout, uint64_t seed, smalgo_mlp** out) {
+  if (!ctx || !out || nin < 1 || nh < 1 || nout < 1) return SMALGO_ERR_ARG;
+  *out = nullptr;
+  return guarded(ctx, [&] {
+    auto* h = new smalgo_mlp{ctx, nullptr};
+    try {
+      h->m = new Mlp(*ctx, nin, nh, nout, seed);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+void smalgo_mlp_free(smalgo_mlp* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->ctx->device);
+  delete h->m;
+  delete h;
+}
+int64_t smalgo_mlp_num_params(const smalgo_mlp* h) { return h ? (int64_t)h->m->np : -1; }
+int smalgo_mlp_get_params(const smalgo_mlp* h, float* p) {
+  if (!h || !p) return SMALGO_ERR_ARG;
+  return guarded(h->ctx, [&] { h->m->get_params(p); });
+}
+int smalgo_mlp_set_params(smalgo_mlp* h, const float* p) {
+  if (!h || !p) return SMALGO_ERR_ARG;
+  return guarded(h->ctx, [&] { h->m->set_params(p); });
+}
+int smalgo_mlp_predict(smalgo_mlp* h, const float* X, int B, float* Y) {
+  if (!h || !X || !Y || B < 1) return SMALGO_ERR_ARG;
+  return guarded(h->ctx, [&] { h->m->predict(X, B, Y); });
+}
+int smalgo_mlp_gradient(smalgo_mlp* h, const float* X, const float* T, int B, float* grad,
+                        float* loss) {
+  if (!h || !X || !T || B < 1) return SMALGO_ERR_ARG;
+  return guarded(h->ctx, [&] {
+    const float l = h->m->gradient(X, T, B, grad);
+    if (loss) *loss = l;
+  });
+}
+int smalgo_mlp_train(smalgo_mlp* h, const float* X, const float* T, int B, int epochs, float lr,
+                     float* loss_hist, float* loss) {
+  if (!h || !X || !T || B < 1 || epochs < 0) return SMALGO_ERR_ARG;
+  return guarded(h->ctx, [&] {
+    const float l = h->m->train(X, T, B, epochs, lr, loss_hist);
+    if (loss) *loss = l;
   });
 }
 

@@ -1,0 +1,170 @@
+"""Learned POLFED filter parameters.
+
+A two-layer perceptron (68 → 128 ReLU → 2), trained and evaluated with the
+library's HIP kernels (``smalgo_mlp_*``), maps 64 KPM moments of the matrix
+plus the target interval (both in the Chebyshev variable), log(nev/s) and the
+log stochastic count to the filter
+resolution κ (degree K = ⌈κ/h⌉, h = interval half-width in the Chebyshev
+variable) and the basis factor f (mmax = ⌈f·nev/s⌉).  The training labels are
+the cheapest (κ, f) of a cost-ordered ladder that reproduces the plain
+method's converged eigenpairs, logged by ``learned_data.py``.
+
+Fallback: when the prediction would give a Krylov basis smaller than twice the
+stochastic eigenvalue count of the interval (the reference extracts 2·nev Ritz
+candidates, ``polfed.jl:935``), the plain parameter rule is used instead.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from .kpm import count as kpm_count
+from .kpm import plain_parameters
+from .solvers import Context, default_context
+
+_FP = C.POINTER(C.c_float)
+
+NMOM = 64
+# safety margins are calibrated per bucket of predicted block steps mmax
+MARGIN_BUCKETS = (10.0, 16.0)
+
+
+def margin_bucket(blocks: float) -> int:
+    return int(sum(blocks >= b for b in MARGIN_BUCKETS))
+KAPPA_RANGE = (3.0, 10.0)
+FACTOR_RANGE = (1.6, 7.0)
+WEIGHTS = Path(__file__).resolve().parent / "data" / "mlp_weights.json"
+
+
+def _fp(a):
+    return a.ctypes.data_as(_FP)
+
+
+class MLP:
+    """Handle on a device-resident perceptron (``smalgo_mlp``)."""
+
+    def __init__(self, nin: int = NMOM + 4, nh: int = 128, nout: int = 2, seed: int = 1,
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.nin, self.nh, self.nout = nin, nh, nout
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.smalgo_mlp_create(self.ctx.h, nin, nh, nout, seed, C.byref(h)))
+        self.h = h
+        self.np = int(self.ctx.lib.smalgo_mlp_num_params(h))
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.np, dtype=np.float32)
+        self.ctx.check(self.ctx.lib.smalgo_mlp_get_params(self.h, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.np
+        self.ctx.check(self.ctx.lib.smalgo_mlp_set_params(self.h, _fp(p)))
+
+    def predict(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], self.nout), dtype=np.float32)
+        self.ctx.check(self.ctx.lib.smalgo_mlp_predict(self.h, _fp(X), X.shape[0], _fp(Y)))
+        return Y
+
+    def gradient(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(self.np, dtype=np.float32)
+        loss = C.c_float()
+        self.ctx.check(self.ctx.lib.smalgo_mlp_gradient(self.h, _fp(X), _fp(T), X.shape[0], _fp(g),
+                                                        C.byref(loss)))
+        return loss.value, g
+
+    def train(self, X, T, epochs: int, lr: float = 1e-3):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        hist = np.zeros(max(epochs, 1), dtype=np.float32)
+        loss = C.c_float()
+        self.ctx.check(self.ctx.lib.smalgo_mlp_train(self.h, _fp(X), _fp(T), X.shape[0], epochs, lr,
+                                                     _fp(hist), C.byref(loss)))
+        return hist[:epochs]
+
+    def save(self, path: Path, meta: dict):
+        Path(path).parent.mkdir(parents=True, exist_ok=True)
+        Path(path).write_text(json.dumps(dict(nin=self.nin, nh=self.nh, nout=self.nout,
+                                              params=self.params().tolist(), **meta)))
+
+    @classmethod
+    def load(cls, path: Path = WEIGHTS, ctx: Optional[Context] = None):
+        d = json.loads(Path(path).read_text())
+        m = cls(d["nin"], d["nh"], d["nout"], ctx=ctx)
+        m.set_params(np.array(d["params"], dtype=np.float32))
+        m.meta = {k: v for k, v in d.items() if k != "params"}
+        return m
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.ctx.lib.smalgo_mlp_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+NIN = NMOM + 4
+
+
+def features(mu: np.ndarray, emin: float, emax: float, smin: float, smax: float, n: int,
+             nev: int, s: int) -> np.ndarray:
+    """68 inputs: the Jackson-damped moments μ_0..μ_63 of Â, the interval
+    (centre and log half-width in the Chebyshev variable), and two quantities
+    derived from those and the request: log(nev/s), the number of block steps
+    per unit basis factor, and log of the stochastic eigenvalue count."""
+    from .solvers import jackson_kernel
+    c0, d = 0.5 * (smin + smax), 0.5 * (smax - smin)
+    a, b = (emin - c0) / d, (emax - c0) / d
+    g = jackson_kernel(NMOM - 1)
+    m = np.asarray(mu[:NMOM], dtype=np.float64) * g
+    h = 0.5 * (b - a)
+    est = max(kpm_count(mu, n, emin, emax, smin, smax), 1.0)
+    return np.concatenate([m, [0.5 * (a + b), math.log(h), math.log(nev / s),
+                               math.log(est)]]).astype(np.float32)
+
+
+def encode_targets(kappa: float, factor: float) -> np.ndarray:
+    return np.array([(kappa - 5.0) / 2.0, (factor - 2.4) / 0.4], dtype=np.float32)
+
+
+def decode_targets(y) -> tuple[float, float]:
+    kappa = float(np.clip(5.0 + 2.0 * y[0], *KAPPA_RANGE))
+    factor = float(np.clip(2.4 + 0.4 * y[1], *FACTOR_RANGE))
+    return kappa, factor
+
+
+def choose_parameters(model: Optional[MLP], mu: np.ndarray, n: int, nev: int, emin: float,
+                      emax: float, smin: float, smax: float, s: int, margin_kappa=None,
+                      margin_factor=None):
+    """(degree, mmax, source) from the network plus its calibrated safety
+    margins, or from the plain rule when the predicted basis undercounts the
+    stochastic estimate (or no model)."""
+    K0, m0 = plain_parameters(n, nev, emin, emax, smin, smax, s)
+    if model is None:
+        return K0, m0, "plain"
+    meta = getattr(model, "meta", {})
+    y = model.predict(features(mu, emin, emax, smin, smax, n, nev, s)[None, :])[0]
+    kappa, factor = decode_targets(y)
+    margins = meta.get("margins")
+    if margins is not None:
+        mk, mf = margins[margin_bucket(factor * nev / s)]
+    else:
+        mk, mf = 0.0, 0.0
+    kappa += mk if margin_kappa is None else margin_kappa
+    factor += mf if margin_factor is None else margin_factor
+    K, mmax = plain_parameters(n, nev, emin, emax, smin, smax, s, kappa, factor)
+    est = kpm_count(mu, n, emin, emax, smin, smax)
+    if mmax * s < 2.0 * est:
+        return K0, m0, "fallback"
+    return K, mmax, "learned"

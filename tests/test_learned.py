@@ -1,0 +1,82 @@
+"""Learned parameter policy on the CPU: target encoding, margin buckets,
+features, and the stochastic-count fallback (with a stand-in predictor)."""
+import numpy as np
+import pytest
+
+from sparse_matrix_algorithms_amd import learned
+from sparse_matrix_algorithms_amd.kpm import plain_parameters
+
+
+def test_target_roundtrip_and_clipping():
+    for k, f in [(5.0, 2.8), (8.0, 4.2), (3.0, 1.6)]:
+        assert np.allclose(learned.decode_targets(learned.encode_targets(k, f)), (k, f))
+    assert learned.decode_targets([100.0, -100.0]) == (learned.KAPPA_RANGE[1],
+                                                       learned.FACTOR_RANGE[0])
+
+
+def test_margin_buckets():
+    assert learned.margin_bucket(5) == 0
+    assert learned.margin_bucket(10) == 1
+    assert learned.margin_bucket(15.9) == 1
+    assert learned.margin_bucket(40) == 2
+
+
+class _Fixed:
+    """stand-in predictor returning fixed encoded targets"""
+
+    def __init__(self, kappa, factor, margins=None):
+        self.y = learned.encode_targets(kappa, factor)
+        self.meta = {"margins": margins} if margins else {}
+
+    def predict(self, X):
+        assert X.shape[1] == learned.NIN
+        return np.tile(self.y, (X.shape[0], 1))
+
+
+def _mu():
+    # moments of a flat density on [-1, 1] (μ_k of the uniform measure)
+    k = np.arange(64)
+    mu = np.zeros(64)
+    mu[0] = 1.0
+    even = (k % 2 == 0) & (k > 0)
+    mu[even] = 1.0 / (1.0 - k[even] ** 2)
+    return mu
+
+
+def test_features_shape_and_fallback():
+    mu = _mu()
+    n, nev, s, smin, smax = 10000, 120, 32, -10.0, 10.0
+    emin, emax = -0.1, 0.1
+    x = learned.features(mu, emin, emax, smin, smax, n, nev, s)
+    assert x.shape == (learned.NIN,) and np.isfinite(x).all()
+    K, mmax, src = learned.choose_parameters(_Fixed(5.0, 3.0), mu, n, nev, emin, emax, smin,
+                                             smax, s)
+    assert src == "learned"
+    assert (K, mmax) == plain_parameters(n, nev, emin, emax, smin, smax, s, 5.0, 3.0)
+    # a basis smaller than twice the stochastic count falls back to the plain rule
+    K2, m2, src2 = learned.choose_parameters(_Fixed(5.0, 1.6), mu, n, 8, -0.5, 0.5, smin, smax,
+                                             s)
+    assert src2 == "fallback"
+    assert (K2, m2) == plain_parameters(n, 8, -0.5, 0.5, smin, smax, s)
+    assert learned.choose_parameters(None, mu, n, nev, emin, emax, smin, smax, s)[2] == "plain"
+
+
+def test_margins_are_applied_per_bucket():
+    mu = _mu()
+    m = _Fixed(5.0, 2.8, margins=[[1.0, 1.0], [0.5, 0.0], [0.0, 0.0]])
+    n, s, smin, smax = 10000, 32, -10.0, 10.0
+    # nev=60: 2.8*60/32 = 5.25 block steps -> bucket 0 -> (6.0, 3.8)
+    K, mmax, _ = learned.choose_parameters(m, mu, n, 60, -0.1, 0.1, smin, smax, s)
+    assert (K, mmax) == plain_parameters(n, 60, -0.1, 0.1, smin, smax, s, 6.0, 3.8)
+
+
+def test_committed_dataset_and_weights():
+    import json
+    recs = [json.loads(l) for l in open(learned.WEIGHTS.parent / "anderson_runs.jsonl")]
+    assert len(recs) == 320 and len({r["index"] for r in recs}) == 320
+    assert all(len(r["mu"]) == 64 for r in recs)
+    w = json.loads(learned.WEIGHTS.read_text())
+    assert w["nin"] == learned.NIN and w["nh"] == 128 and w["nout"] == 2
+    assert len(w["params"]) == learned.NIN * 128 + 128 + 128 * 2 + 2
+    ev = json.loads((learned.WEIGHTS.parent / "heldout_eval.json").read_text())["summary"]
+    assert ev["instances"] == 64 and ev["success_learned"] >= ev["success_plain"]
