@@ -59,6 +59,10 @@ def parse():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-learned", action="store_true", help="plain parameter rule only")
+    ap.add_argument("--headline-fp64", action="store_true", help="time the FP64 solve instead")
+    ap.add_argument("--skip-variants", action="store_true",
+                    help="skip the FP64-learned and mixed-plain comparison runs")
     ap.add_argument("--profile", action="store_true",
                     help="one mixed solve, no FP64/CPU legs (for rocprofv3 runs)")
     return ap.parse_args()
@@ -83,6 +87,7 @@ def main():
         sys.exit(2)
     if args.profile:
         args.steps, args.warmup, args.skip_fp64, args.skip_cpu = 1, 0, True, True
+        args.skip_variants = True
 
     import sparse_matrix_algorithms_amd as sm
     from sparse_matrix_algorithms_amd import kpm
@@ -105,27 +110,33 @@ def main():
     t_k = time.time() - t_k
     K, mmax = kpm.plain_parameters(n, args.nev, emin, emax, smin, smax, args.s, args.kappa,
                                    args.mmax_factor)
+    from sparse_matrix_algorithms_amd.learned import MLP, choose_parameters
+    model = None if args.no_learned else MLP.load(ctx=ctx)
+    KL, mmaxL, src = choose_parameters(model, mu[:64], n, args.nev, emin, emax, smin, smax, args.s)
     log(f"[bench] KPM({args.kpm_moments} moments, {args.kpm_probes} probes) interval "
-        f"[{emin:.6f},{emax:.6f}] in {t_k:.2f}s; plain rule: degree={K} mmax={mmax}")
+        f"[{emin:.6f},{emax:.6f}] in {t_k:.2f}s; plain rule: degree={K} mmax={mmax}; "
+        f"{src}: degree={KL} mmax={mmaxL}")
 
-    common = dict(s=args.s, smin=smin, smax=smax, sigma=0.0, emin=emin, emax=emax, mmax=mmax,
+    common = dict(s=args.s, smin=smin, smax=smax, sigma=0.0, emin=emin, emax=emax,
                   tol=1e-10 * anorm, ctx=ctx, return_stats=True, seed=1)
 
-    def solve(precision):
+    def solve(precision, degree, mm):
         extra = {}
         if precision == "fp32":
             extra = dict(refine_guard=args.refine_guard, refine_degree=args.refine_degree,
                          refine_maxiter=args.refine_maxiter)
         t = time.perf_counter()
-        lam, X, res, st = sm.polfed(dm, args.nev, K, precision=precision, **common, **extra)
+        lam, X, res, st = sm.polfed(dm, args.nev, degree, mmax=mm, precision=precision, **common,
+                                    **extra)
         return lam, X, res, st, time.perf_counter() - t
 
+    head_prec = "fp64" if args.headline_fp64 else "fp32"
     for w in range(args.warmup):
-        _, _, _, st, dt = solve("fp32")
+        _, _, _, st, dt = solve(head_prec, KL, mmaxL)
         log(f"[bench] warmup {w}: {dt:.3f}s {st['line']}")
     times, last = [], None
     for k in range(args.steps):
-        lam, X, res, st, dt = solve("fp32")
+        lam, X, res, st, dt = solve(head_prec, KL, mmaxL)
         times.append(dt)
         last = (lam, res, st)
         log(f"[bench] step {k}: {dt:.3f}s {st['line']} t_filter={st['t_filter']:.3f} "
@@ -135,44 +146,56 @@ def main():
     nvalid, ok = valid_pairs(lam, res, emin, emax, anorm)
     value = nvalid / t_step
 
-    detail = dict(n=n, nnz=dm.nnz, interval=[emin, emax], degree=K, mmax=mmax, s=args.s,
+    def summary(lam_, res_, st_, dt_, K_, mm_):
+        nv, ok_ = valid_pairs(lam_, res_, emin, emax, anorm)
+        return dict(seconds=round(dt_, 4), degree=K_, mmax=mm_, pairs=int(len(lam_)), valid=nv,
+                    eigenpairs_per_s=round(nv / dt_, 3),
+                    max_rel_res=float(res_.max() / anorm) if len(res_) else None,
+                    t_filter=st_["t_filter"], t_orth=st_["t_orth"], t_extract=st_["t_extract"],
+                    t_refine=st_.get("t_refine", 0.0),
+                    refine_filter_steps=st_.get("refine_filter_steps", 0),
+                    locked=st_.get("locked", 0),
+                    filter_gbs=round(st_["spmm_bytes"] / st_["t_filter"] / 1e9, 1),
+                    orth_gemm_tflops=round(st_["gemm_flops"] / max(st_["t_orth"], 1e-9) / 1e12,
+                                           3)), np.sort(lam_[ok_])
+
+    head, head_set = summary(lam, res, st, t_step, KL, mmaxL)
+    detail = dict(n=n, nnz=dm.nnz, interval=[emin, emax], s=args.s, parameter_source=src,
                   kpm_seconds=round(t_k, 3), upload_seconds=round(t_up, 3),
-                  mixed=dict(seconds=round(t_step, 4), pairs=int(len(lam)), valid=nvalid,
-                             max_rel_res=float(res.max() / anorm) if len(res) else None,
-                             t_filter=st["t_filter"], t_orth=st["t_orth"],
-                             t_extract=st["t_extract"], t_refine=st.get("t_refine", 0.0),
-                             refine_filter_steps=st.get("refine_filter_steps", 0),
-                             locked=st.get("locked", 0)))
-    # dominant kernel: the FP32 fused Clenshaw step of the basis build
+                  headline=dict(precision=head_prec, **head))
+    # dominant kernel: the fused Clenshaw step of the basis build
     launches = st["filter_steps"]
     avg_launch = st["t_filter"] / max(launches, 1)
     bytes_per_launch = st["spmm_bytes"] / max(launches, 1)
     achieved = bytes_per_launch / avg_launch / 1e9
+    tname = "float" if head_prec == "fp32" else "double"
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel=(f"clenshaw_kernel<float,{int(st['tile_vec'])},{int(st['tile_lpr'])},"
+                    kernel=(f"clenshaw_kernel<{tname},{int(st['tile_vec'])},{int(st['tile_lpr'])},"
                             f"{int(st['tile_rt'])},false>"), bytes_per_launch=bytes_per_launch,
                     avg_launch_us=round(avg_launch * 1e6, 3), launches=int(launches))
-    gemm_s = st["t_orth"]
-    detail["mixed"]["orth_gemm_tflops"] = round(st["gemm_flops"] / max(gemm_s, 1e-9) / 1e12, 3)
+
+    def same(a, b):
+        return bool(len(a) == len(b) and (len(a) == 0 or
+                                          float(np.max(np.abs(a - b))) <= 1e-10 * anorm))
 
     if not args.skip_fp64:
-        lam64, X64, res64, st64, dt64 = solve("fp64")
-        n64, ok64 = valid_pairs(lam64, res64, emin, emax, anorm)
-        a = np.sort(lam[ok])
-        b = np.sort(lam64[ok64])
-        same = len(a) == len(b) and (len(a) == 0 or float(np.max(np.abs(a - b))) <= 1e-10 * anorm)
-        detail["fp64"] = dict(seconds=round(dt64, 4), pairs=int(len(lam64)), valid=n64,
-                              eigenpairs_per_s=round(n64 / dt64, 3),
-                              max_rel_res=float(res64.max() / anorm) if len(res64) else None,
-                              t_filter=st64["t_filter"], t_orth=st64["t_orth"],
-                              t_extract=st64["t_extract"], t_refine=st64.get("t_refine", 0.0),
-                              filter_gbs=round(st64["spmm_bytes"] / st64["t_filter"] / 1e9, 1))
-        detail["same_eigenvalue_set"] = bool(same)
-        detail["max_abs_dlambda"] = (float(np.max(np.abs(a - b))) if len(a) == len(b) and len(a)
-                                     else None)
-        detail["speedup_mixed_vs_fp64"] = round((nvalid / t_step) / max(n64 / dt64, 1e-12), 3)
-        log(f"[bench] fp64: {dt64:.3f}s {st64['line']} valid={n64} same_set={same}")
+        lam64, X64, res64, st64, dt64 = solve("fp64", K, mmax)
+        detail["fp64_plain"], set64 = summary(lam64, res64, st64, dt64, K, mmax)
+        detail["same_eigenvalue_set_as_fp64_plain"] = same(head_set, set64)
+        detail["speedup_vs_fp64_plain"] = round(value / max(detail["fp64_plain"]
+                                                            ["eigenpairs_per_s"], 1e-12), 3)
+        log(f"[bench] fp64 plain: {dt64:.3f}s {st64['line']}")
+        st64_steps = st64["filter_steps"]
+        if not args.skip_variants:
+            l2, _, r2, s2, d2 = solve("fp64", KL, mmaxL)
+            detail["fp64_" + src], set2 = summary(l2, r2, s2, d2, KL, mmaxL)
+            detail["fp64_" + src]["same_set_as_fp64_plain"] = same(set2, set64)
+            log(f"[bench] fp64 {src}: {d2:.3f}s {s2['line']}")
+            l3, _, r3, s3, d3 = solve("fp32", K, mmax)
+            detail["mixed_plain"], set3 = summary(l3, r3, s3, d3, K, mmax)
+            detail["mixed_plain"]["same_set_as_fp64_plain"] = same(set3, set64)
+            log(f"[bench] mixed plain: {d3:.3f}s {s3['line']}")
 
     cpu = None
     if not args.skip_cpu:
@@ -190,7 +213,7 @@ def main():
             steps += len(kpm_co)
         per_step = t_cpu / steps
         # reference FP64 algorithm: mmax·K filter steps per solve at this block width
-        steps64 = (detail.get("fp64", {}) and st64["filter_steps"]) or st["filter_steps"]
+        steps64 = st64_steps if "fp64_plain" in detail else st["filter_steps"]
         t_cpu_solve = per_step * steps64
         cpu = dict(value=round(nvalid / t_cpu_solve, 4), unit="eigenpairs/s", cores=threads,
                    kind="port",
@@ -211,12 +234,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64",
+        "dtype": "fp32+fp64" if head_prec == "fp32" else "fp64",
         "data": "synthetic (seeded 3D Anderson disorder)",
         "config": {"workload": f"POLFED 3D Anderson L={args.L} W={args.W} n={n}, ~{args.nev} "
-                               f"eigenpairs at E=0, s={args.s}, FP32 filter + FP64 refine",
-                   "L": args.L, "n": n, "nev": args.nev, "block": args.s, "degree": K,
-                   "mmax": mmax, "parallelism": "single GPU"},
+                               f"eigenpairs at E=0, s={args.s}, "
+                               + ("FP64" if head_prec == "fp64" else "FP32 filter + FP64 refine")
+                               + f", {src} filter parameters",
+                   "L": args.L, "n": n, "nev": args.nev, "block": args.s, "degree": KL,
+                   "mmax": mmaxL, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": detail,

@@ -326,43 +326,43 @@ void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::
     const int pw = panel_width<T>();
     const int npan = (b + pw - 1) / pw;
     const int n = op.n;
-    const size_t sz = (size_t)n * npan * pw;
-    DBuf<T> Qp(sz), P0(sz), P1(sz), Yp(sz);
-    to_panels<T>(cx.stream, n, b, pw, Q, ldq, Qp.p);
-    // one launch per panel and step: each launch has the shape of the
-    // narrow-block filter (the tile the autotuner picks for width pw), which
-    // runs faster per byte than one launch spanning all panels
     const size_t ps = (size_t)n * pw;
+    DBuf<T> Qp(ps * npan), P0(ps), P1(ps), Yp(ps * npan);
+    to_panels<T>(cx.stream, n, b, pw, Q, ldq, Qp.p);
+    // Panels run one after another through the whole recurrence (panel-outer,
+    // step-inner): one panel's Q, B1, B2 (3 x n x 512 B) stay resident in the
+    // 256 MB Infinity Cache across its K steps, which interleaving the panels
+    // step by step would evict.  Each launch has the narrow-block shape the
+    // autotuner measured.
     FilterTile tile{};
     if (!env_tile("SMALGO_PANEL_TILE", &tile))
       tile = autotune_tile<T>(cx, op, pw, Qp.p, pw, P0.p, P1.p, pw, Yp.p, pw);
-    auto step = [&](double cq, double a1, double b1, FBlk<T> X1, FBlk<T> X2, T* out) {
-      const T* x1p = X1.p;
-      double a1e = a1, b1e = b1;
-      if (!x1p) {
-        if (X1.qs == 0.0) {
-          a1e = b1e = 0.0;
-        } else {
-          x1p = Qp.p;
-          a1e = a1 * X1.qs;
-          b1e = b1 * X1.qs;
+    for (int pn = 0; pn < npan; ++pn) {
+      const int bp = std::min(pw, b - pn * pw);
+      const T* Qn = Qp.p + (size_t)pn * ps;
+      auto step = [&](double cq, double a1, double b1, FBlk<T> X1, FBlk<T> X2, T* out) {
+        const T* x1p = X1.p;
+        double a1e = a1, b1e = b1;
+        if (!x1p) {
+          if (X1.qs == 0.0) {
+            a1e = b1e = 0.0;
+          } else {
+            x1p = Qn;
+            a1e = a1 * X1.qs;
+            b1e = b1 * X1.qs;
+          }
         }
-      }
-      const T* x2p = X2.p;
-      double cqe = cq, s2e = 1.0;
-      if (!x2p) {
-        cqe = cq - X2.qs;
-        s2e = 0.0;
-      }
-      for (int pn = 0; pn < npan; ++pn) {
-        const int bp = std::min(pw, b - pn * pw);
-        const size_t o = (size_t)pn * ps;
-        launch_clenshaw_step<T>(cx.stream, *op.A, bp, Qp.p + o, pw, (T)cqe, x1p ? x1p + o : nullptr,
-                                pw, (T)a1e, (T)b1e, x2p ? x2p + o : nullptr, pw, (T)s2e, out + o,
-                                pw, tile);
-      }
-    };
-    clenshaw_recurrence<T>(co, smin, smax, P0.p, P1.p, Yp.p, step);
+        const T* x2p = X2.p;
+        double cqe = cq, s2e = 1.0;
+        if (!x2p) {
+          cqe = cq - X2.qs;
+          s2e = 0.0;
+        }
+        launch_clenshaw_step<T>(cx.stream, *op.A, bp, Qn, pw, (T)cqe, x1p, pw, (T)a1e, (T)b1e,
+                                x2p, pw, (T)s2e, out, pw, tile);
+      };
+      clenshaw_recurrence<T>(co, smin, smax, P0.p, P1.p, Yp.p + (size_t)pn * ps, step);
+    }
     from_panels<T>(cx.stream, n, b, pw, Yp.p, Y, ldy);
     return;
   }
