@@ -169,10 +169,17 @@ def main():
     bytes_per_launch = st["spmm_bytes"] / max(launches, 1)
     achieved = bytes_per_launch / avg_launch / 1e9
     tname = "float" if head_prec == "fp32" else "double"
+    kname = (f"clenshaw_kernel<{tname},{int(st['tile_vec'])},{int(st['tile_lpr'])},"
+             f"{int(st['tile_rt'])},false>")
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs of
+    # the same kernel and tile on this workload (scaled by the copy-kernel calibration)
+    traffic = None
+    tfile = ROOT / "profiles" / "clenshaw_anderson_L48_b64_traffic.json"
+    if tfile.exists() and args.L == 48 and args.s == 64:
+        traffic = json.loads(tfile.read_text()).get(kname, {}).get("traffic_bytes")
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel=(f"clenshaw_kernel<{tname},{int(st['tile_vec'])},{int(st['tile_lpr'])},"
-                            f"{int(st['tile_rt'])},false>"), bytes_per_launch=bytes_per_launch,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                    kernel=kname, bytes_per_launch=bytes_per_launch,
                     avg_launch_us=round(avg_launch * 1e6, 3), launches=int(launches))
 
     def same(a, b):
