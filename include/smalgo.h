@@ -215,6 +215,45 @@ int smalgo_polfed(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn, v
                   int64_t n, int nev, int degree, const smalgo_polfed_params* p,
                   const double* Q1, smalgo_result** out);
 
+/* ---- batched spectral slices (no reference counterpart) -----------------
+ * The reference solves one interval per polfed call (polfed.jl:1128).  Here
+ * many intervals of one matrix run together: slice i is
+ *   polfed(A, nev_i, degree_i; emin=emin_i, emax=emax_i, sigma=sigma_i, s=s_i, mmax=mmax_i, ...)
+ * with every other option from p (window, tol, precision, refinement, ...),
+ * start block from seed p->seed + i.  At each block step the filters of all
+ * active slices share one packed launch sequence: their blocks are packed
+ * column-wise into dense panels with per-column Chebyshev coefficients
+ * (slices of lower degree wait through leading zero coefficients), and the
+ * sequence is captured in HIP graphs once and replayed (use_graph = 1) or
+ * launched eagerly (0).  SMALGO_SLICE_PW fixes the panel width. */
+typedef struct smalgo_slice {
+  double emin, emax;  /* target interval */
+  double sigma;       /* window centre (delta window) */
+  int nev;            /* eigenpairs wanted */
+  int s;              /* block width of this slice */
+  int degree;         /* filter degree K */
+  int mmax;           /* basis blocks; <= 0: ceil(2.8 nev / s) */
+} smalgo_slice;
+
+/* out: array of nslices result handles (each freed with smalgo_result_free);
+ * t_total / t_filter / t_orth statistics are those of the whole batch. */
+int smalgo_polfed_slices(smalgo_ctx* ctx, const smalgo_mat* A, int nslices,
+                         const smalgo_slice* slices, const smalgo_polfed_params* p, int use_graph,
+                         smalgo_result** out);
+
+/* The packed multi-slice filter alone: Y_i = p_i(Â) Q_i for nslices blocks of
+ * widths[i] columns, coefficients concatenated (degrees[i] + 1 each).  Q and Y
+ * hold `replays` consecutive host blocks, each n x Σwidths column-major
+ * (leading dimension ldq / ldy, slice i at the running column offset); block r
+ * goes through the same plan (and, with use_graph, the same instantiated
+ * graphs) as block 0.  panel_width 0 picks it by timing.  stats (optional,
+ * 6 doubles): panel width, panels, launches per run, ms per run, modelled
+ * bytes per run, graphs per run. */
+int smalgo_apply_poly_slices(smalgo_ctx* ctx, const smalgo_mat* A, int nslices, const int* widths,
+                             const int* degrees, const double* coeffs, double smin, double smax,
+                             int precision, int panel_width, int use_graph, int replays,
+                             const double* Q, int64_t ldq, double* Y, int64_t ldy, double* stats);
+
 /* ---- Chebyshev filtered subspace iteration ------------------------------ */
 typedef struct smalgo_cheb_params {
   int has_bounds;

@@ -37,6 +37,13 @@ class PolfedParams(C.Structure):
     ]
 
 
+class SliceParams(C.Structure):
+    _fields_ = [
+        ("emin", C.c_double), ("emax", C.c_double), ("sigma", C.c_double), ("nev", C.c_int),
+        ("s", C.c_int), ("degree", C.c_int), ("mmax", C.c_int),
+    ]
+
+
 class ChebParams(C.Structure):
     _fields_ = [
         ("has_bounds", C.c_int), ("smin", C.c_double), ("smax", C.c_double),
@@ -103,6 +110,12 @@ SIGNATURES = [
     ("smalgo_polfed", C.c_int,
      [_P, _P, MATBLOCK_FN, _P, C.c_int64, C.c_int, C.c_int, C.POINTER(PolfedParams), _DP,
       C.POINTER(_P)]),
+    ("smalgo_polfed_slices", C.c_int,
+     [_P, _P, C.c_int, C.POINTER(SliceParams), C.POINTER(PolfedParams), C.c_int,
+      C.POINTER(_P)]),
+    ("smalgo_apply_poly_slices", C.c_int,
+     [_P, _P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _DP, C.c_double, C.c_double,
+      C.c_int, C.c_int, C.c_int, C.c_int, _DP, C.c_int64, _DP, C.c_int64, _DP]),
     ("smalgo_cheb_default_params", None, [C.POINTER(ChebParams)]),
     ("smalgo_chebyshev", C.c_int,
      [_P, _P, MATBLOCK_FN, _P, C.c_int64, C.c_double, C.c_double, C.POINTER(ChebParams), _DP,

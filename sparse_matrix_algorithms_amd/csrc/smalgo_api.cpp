@@ -531,6 +531,52 @@ int smalgo_polfed(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn, v
   });
 }
 
+int smalgo_polfed_slices(smalgo_ctx* ctx, const smalgo_mat* A, int nslices,
+                         const smalgo_slice* slices, const smalgo_polfed_params* p, int use_graph,
+                         smalgo_result** out) {
+  if (!ctx || !A || !out || !slices || nslices < 1) return SMALGO_ERR_ARG;
+  for (int i = 0; i < nslices; ++i) out[i] = nullptr;
+  return guarded(ctx, [&] {
+    Op op;
+    if (!make_op(A, nullptr, nullptr, A->A.n, op)) throw Error(-1, "bad matrix");
+    smalgo_polfed_params prm;
+    if (p)
+      prm = *p;
+    else
+      smalgo_polfed_default_params(&prm);
+    std::vector<SliceSpec> sl(nslices);
+    for (int i = 0; i < nslices; ++i)
+      sl[i] = SliceSpec{slices[i].emin, slices[i].emax, slices[i].sigma, slices[i].nev,
+                        slices[i].s,    slices[i].degree, slices[i].mmax};
+    std::vector<Result*> rs = polfed_slices_solve(*ctx, op, sl, prm, use_graph != 0);
+    for (int i = 0; i < nslices; ++i) out[i] = handle_of(rs[i]);
+  });
+}
+
+int smalgo_apply_poly_slices(smalgo_ctx* ctx, const smalgo_mat* A, int nslices, const int* widths,
+                             const int* degrees, const double* coeffs, double smin, double smax,
+                             int precision, int panel_width, int use_graph, int replays,
+                             const double* Q, int64_t ldq, double* Y, int64_t ldy, double* stats) {
+  if (!ctx || !A || nslices < 1 || !widths || !degrees || !coeffs || !Q || !Y || replays < 1)
+    return SMALGO_ERR_ARG;
+  return guarded(ctx, [&] {
+    Op op;
+    if (!make_op(A, nullptr, nullptr, A->A.n, op)) throw Error(-1, "bad matrix");
+    if (ldq < op.n || ldy < op.n) throw Error(-1, "leading dimension smaller than n");
+    if (!(smax > smin)) throw Error(-1, "need smin < smax");
+    std::vector<int> w(widths, widths + nslices);
+    std::vector<std::vector<double>> co(nslices);
+    size_t off = 0;
+    for (int i = 0; i < nslices; ++i) {
+      if (degrees[i] < 0) throw Error(-1, "degrees must be nonnegative");
+      co[i].assign(coeffs + off, coeffs + off + degrees[i] + 1);
+      off += (size_t)degrees[i] + 1;
+    }
+    apply_poly_slices_host(*ctx, op, w, co, smin, smax, precision, panel_width, use_graph != 0,
+                           replays, Q, (int)ldq, Y, (int)ldy, stats);
+  });
+}
+
 int smalgo_chebyshev(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn, void* user,
                      int64_t n, double emin, double emax, const smalgo_cheb_params* p,
                      const double* Q0, smalgo_result** out) {

@@ -766,136 +766,156 @@ void kpm_moments(Ctx& cx, const Op& op, int nmom, int R, uint64_t seed, double s
 }
 
 // ---------------------------------------------------------------------------
-// POLFED
+// POLFED building blocks
 // ---------------------------------------------------------------------------
+// Block basis of one interval: V = [Q_1 … Q_m] row-major n x ldV (block j at
+// columns [j·s, j·s + r_j)), with the reduced-matrix blocks A_j, B_j.
 template <typename T>
-static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smalgo_polfed_params& p,
-                          const double* Q1host) {
-  using clk = std::chrono::steady_clock;
-  const auto t0 = clk::now();
-  const int n = op.n;
-  if (nev < 1) throw Error(-1, "nev must be positive");
-  if (degree < 0) throw Error(-1, "degree must be nonnegative");
-  if (p.s < 1) throw Error(-1, "block size s must be positive");
-  if (p.s > n) throw Error(-1, "block size s must not exceed n");
-  double smin = p.smin, smax = p.smax;
-  if (!p.has_bounds) {
-    if (op.is_callback()) throw Error(-1, "Matrix-free POLFED requires spectral bounds smin and smax");
-    smin = op.A->gersh_min;
-    smax = op.A->gersh_max;
-  }
-  if (!(smax > smin)) throw Error(-1, "need smin < smax");
-  const std::vector<double> coeffs =
-      build_coeffs(degree, smin, smax, p.sigma, p.emin, p.emax, p.window_type, p.jackson != 0);
-  const int s = p.s;
-  const int mmax = p.mmax > 0 ? p.mmax : (int)std::ceil(2.8 * nev / (double)s);
-  const int lds = round_up(s, 4);
-  const int ldV = round_up(mmax * s, 4);
-  PhaseTimer tm(cx.stream);
-  Dense dn(cx);
-  OrthWork ow;
-
-  DBuf<T> V((size_t)n * ldV), W((size_t)n * lds), b0((size_t)n * lds), b1((size_t)n * lds);
-  DBuf<double> pbuf;
-  if (op.is_callback()) pbuf.alloc((size_t)n * lds);
-  fill_block<T>(cx.stream, n, ldV, V.p, ldV, T(0));
-  // ---- start block: orth_block_polfed!(randn(n,s); rtol) (polfed.jl:1133-1134)
-  int r_cur = 0;
-  {
-    DBuf<double> q64((size_t)n * lds), t64((size_t)n * lds), o64((size_t)n * lds);
-    if (Q1host) {
-      DBuf<double> cm((size_t)n * s);
-      SMALGO_HIP(hipMemcpyAsync(cm.p, Q1host, sizeof(double) * (size_t)n * s,
-                                hipMemcpyHostToDevice, cx.stream));
-      cm_to_rm<double, double>(cx.stream, n, s, cm.p, n, q64.p, lds);
-      SMALGO_HIP(hipStreamSynchronize(cx.stream));
-    } else {
-      gaussian_block<double>(cx.stream, n, s, p.seed, q64.p, lds);
-    }
-    r_cur = cholqr<double>(cx, dn, ow, n, s, q64.p, lds, o64.p, lds, t64.p, lds, p.rtol, nullptr);
-    copy_block<double, T>(cx.stream, n, r_cur, o64.p, lds, V.p, ldV);
-  }
+struct Basis {
+  DBuf<T> V;
+  int ldV = 0, s = 0, mmax = 0;
   std::vector<int> ranks;
   std::vector<std::vector<double>> Ablk, Bblk;  // column-major r_j x r_j, r_{j+1} x r_j
-  DBuf<double> Adev((size_t)s * s), Bprev((size_t)s * s);
-  DBuf<T> Cbuf((size_t)ldV * lds);
-  size_t gwn = 0;
-  for (int k = 1; k <= mmax; ++k) gwn = std::max(gwn, gemm_tn_work_elems(n, k * s, s));
-  DBuf<double> gw(gwn);
-  std::vector<double> hA((size_t)s * s), R;
-  int r_prev = 0;
-  FilterTile basis_tile{};
-  double t_filter = 0;
-  long long filter_steps = 0;
-  double gemm_flops = 0;
-  for (int j = 0; j < mmax && r_cur > 0; ++j) {
-    const int r = r_cur;
-    ranks.push_back(r);
-    T* Qj = V.p + (size_t)j * s;
-    size_t hf = tm.begin("filter");
-    apply_filter<T>(cx, op, r, Qj, ldV, coeffs, smin, smax, W.p, lds, b0.p, b1.p, lds, pbuf.p);
-    tm.end(hf);
-    if (j == 0) basis_tile = cx.last_tile;
-    filter_steps += degree;  // K fused steps (the all-zero first step is skipped)
-    size_t ho = tm.begin("orth");
-    // W -= Q_{j-1} B_{j-1}ᵀ   (polfed.jl:1073-1075)
-    if (j > 0 && r_prev > 0) {
-      const T* Qp = V.p + (size_t)(j - 1) * s;
-      gemm_nn<T, double, T>(cx.stream, n, r_prev, r, -1.0, Qp, ldV, Bprev.p, s, false, 1.0, W.p,
-                            lds);
-      gemm_flops += 2.0 * n * r_prev * r;
-    }
-    // A_j = sym(Q_jᵀ W)   (polfed.jl:1078)
-    gemm_tn<T, T, double>(cx.stream, n, r, r, 1.0, Qj, ldV, W.p, lds, 0.0, Adev.p, r, gw.p, gw.n);
-    symmetrize(cx.stream, r, Adev.p, r);
-    gemm_flops += 2.0 * n * r * r;
-    std::vector<double> Aj((size_t)r * r);
-    SMALGO_HIP(hipMemcpyAsync(Aj.data(), Adev.p, sizeof(double) * r * r, hipMemcpyDeviceToHost,
+  DBuf<double> Adev, Bprev;                     // s x s device blocks
+  int r_cur = 0, r_prev = 0;
+  bool done = false;
+};
+
+struct StepWork {
+  OrthWork ow;
+  DBuf<double> gw;
+  std::vector<double> R;
+};
+
+// orth_block_polfed!(randn(n,s); rtol) (polfed.jl:1133-1134) into block 0
+template <typename T>
+static void basis_start(Ctx& cx, Dense& dn, OrthWork& ow, Basis<T>& B, int n, const double* Q1host,
+                        uint64_t seed, double rtol) {
+  const int s = B.s, lds = round_up(s, 4);
+  DBuf<double> q64((size_t)n * lds), t64((size_t)n * lds), o64((size_t)n * lds);
+  if (Q1host) {
+    DBuf<double> cm((size_t)n * s);
+    SMALGO_HIP(hipMemcpyAsync(cm.p, Q1host, sizeof(double) * (size_t)n * s, hipMemcpyHostToDevice,
                               cx.stream));
+    cm_to_rm<double, double>(cx.stream, n, s, cm.p, n, q64.p, lds);
     SMALGO_HIP(hipStreamSynchronize(cx.stream));
-    Ablk.push_back(std::move(Aj));
-    if (j == mmax - 1) {  // Q_{mmax+1} and B_mmax never enter T or the Ritz vectors
-      tm.end(ho);
-      break;
-    }
-    // W -= Q_j A_j   (polfed.jl:1082)
-    gemm_nn<T, double, T>(cx.stream, n, r, r, -1.0, Qj, ldV, Adev.p, r, false, 1.0, W.p, lds);
-    gemm_flops += 2.0 * n * r * r;
-    // reorth_full!(W, Qhist, Q): against every stored block, then once more
-    // against Q_j (polfed.jl:623-631; Qhist already holds Q_j, 1067)
-    const int mall = (j + 1) * s;
-    for (int pass = 0; pass < p.reorth_passes; ++pass) {
-      gemm_tn<T, T, T>(cx.stream, n, mall, r, 1.0, V.p, ldV, W.p, lds, 0.0, Cbuf.p, lds, gw.p,
-                       gw.n);
-      gemm_nn<T, T, T>(cx.stream, n, mall, r, -1.0, V.p, ldV, Cbuf.p, lds, false, 1.0, W.p, lds);
-      gemm_tn<T, T, T>(cx.stream, n, r, r, 1.0, Qj, ldV, W.p, lds, 0.0, Cbuf.p, lds, gw.p, gw.n);
-      gemm_nn<T, T, T>(cx.stream, n, r, r, -1.0, Qj, ldV, Cbuf.p, lds, false, 1.0, W.p, lds);
-      gemm_flops += 4.0 * n * mall * r + 4.0 * n * r * r;
-    }
-    // [Q_{j+1}, B_j] = block_qr_factor(W; rtol)   (polfed.jl:1090)
-    T* Qn = V.p + (size_t)(j + 1) * s;
-    const int rn = cholqr<T>(cx, dn, ow, n, r, W.p, lds, Qn, ldV, b0.p, lds, p.rtol, &R);
-    gemm_flops += 2.0 * 2.0 * n * r * r * 2;
-    tm.end(ho);
-    if (rn == 0) break;  // rank collapse (polfed.jl:1091)
-    if (rn < r) fill_block<T>(cx.stream, n, s - rn, Qn + rn, ldV, T(0));
-    // B_j = R[1:rn, 1:r]  (column-major, stored with ld s on the device)
-    std::vector<double> Bj((size_t)rn * r);
-    for (int c = 0; c < r; ++c)
-      for (int i = 0; i < rn; ++i) Bj[(size_t)c * rn + i] = R[(size_t)c * r + i];
-    // device copy as row-major-readable Bᵀ: element (k, i) at k*s + i = B(i, k)
-    std::vector<double> Bt((size_t)s * s, 0.0);
-    for (int i = 0; i < rn; ++i)
-      for (int k = 0; k < r; ++k) Bt[(size_t)k * s + i] = Bj[(size_t)k * rn + i];
-    SMALGO_HIP(hipMemcpyAsync(Bprev.p, Bt.data(), sizeof(double) * s * s, hipMemcpyHostToDevice,
-                              cx.stream));
-    SMALGO_HIP(hipStreamSynchronize(cx.stream));
-    Bblk.push_back(std::move(Bj));
-    r_prev = r;
-    r_cur = rn;
+  } else {
+    gaussian_block<double>(cx.stream, n, s, seed, q64.p, lds);
   }
+  B.r_cur = cholqr<double>(cx, dn, ow, n, s, q64.p, lds, o64.p, lds, t64.p, lds, rtol, nullptr);
+  copy_block<double, T>(cx.stream, n, B.r_cur, o64.p, lds, B.V.p, B.ldV);
+  if (B.r_cur == 0) B.done = true;
+}
+
+template <typename T>
+static void basis_alloc(Basis<T>& B, int n, int s, int mmax) {
+  B.s = s;
+  B.mmax = mmax;
+  B.ldV = round_up(mmax * s, 4);
+  B.V.alloc((size_t)n * B.ldV);
+  B.Adev.alloc((size_t)s * s);
+  B.Bprev.alloc((size_t)s * s);
+}
+
+// One block-Lanczos step after the filter W = p(Â) Q_j (polfed.jl:1066-1097):
+// W -= Q_{j-1} B_{j-1}ᵀ; A_j = sym(Q_jᵀ W); W -= Q_j A_j; reorth_full!;
+// [Q_{j+1}, B_j] = block_qr_factor(W).  W (n x r, ld ldw) is overwritten; tmp
+// is an n x (ld ldt) scratch block.  Marks the basis done on rank collapse or
+// after block mmax.
+template <typename T>
+static void lanczos_step(Ctx& cx, Dense& dn, StepWork& sw, const smalgo_polfed_params& p, int n,
+                         Basis<T>& B, int j, T* W, int ldw, T* tmp, int ldt, DBuf<T>& Cbuf,
+                         double& gemm_flops) {
+  const int s = B.s, ldV = B.ldV, r = B.r_cur, r_prev = B.r_prev;
+  T* Qj = B.V.p + (size_t)j * s;
+  if (sw.gw.n < gemm_tn_work_elems(n, (j + 1) * s, r) || sw.gw.n < gemm_tn_work_elems(n, r, r)) {
+    size_t g = 0;
+    for (int k = 1; k <= B.mmax; ++k) g = std::max(g, gemm_tn_work_elems(n, k * s, s));
+    sw.gw.ensure(g);
+  }
+  // W -= Q_{j-1} B_{j-1}ᵀ   (polfed.jl:1073-1075)
+  if (j > 0 && r_prev > 0) {
+    const T* Qp = B.V.p + (size_t)(j - 1) * s;
+    gemm_nn<T, double, T>(cx.stream, n, r_prev, r, -1.0, Qp, ldV, B.Bprev.p, s, false, 1.0, W, ldw);
+    gemm_flops += 2.0 * n * r_prev * r;
+  }
+  // A_j = sym(Q_jᵀ W)   (polfed.jl:1078)
+  gemm_tn<T, T, double>(cx.stream, n, r, r, 1.0, Qj, ldV, W, ldw, 0.0, B.Adev.p, r, sw.gw.p,
+                        sw.gw.n);
+  symmetrize(cx.stream, r, B.Adev.p, r);
+  gemm_flops += 2.0 * n * r * r;
+  std::vector<double> Aj((size_t)r * r);
+  SMALGO_HIP(hipMemcpyAsync(Aj.data(), B.Adev.p, sizeof(double) * r * r, hipMemcpyDeviceToHost,
+                            cx.stream));
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  B.Ablk.push_back(std::move(Aj));
+  B.ranks.push_back(r);
+  if (j == B.mmax - 1) {  // Q_{mmax+1} and B_mmax never enter T or the Ritz vectors
+    B.done = true;
+    return;
+  }
+  // W -= Q_j A_j   (polfed.jl:1082)
+  gemm_nn<T, double, T>(cx.stream, n, r, r, -1.0, Qj, ldV, B.Adev.p, r, false, 1.0, W, ldw);
+  gemm_flops += 2.0 * n * r * r;
+  // reorth_full!(W, Qhist, Q): against every stored block, then once more
+  // against Q_j (polfed.jl:623-631; Qhist already holds Q_j, 1067)
+  const int mall = (j + 1) * s;
+  const int ldc = round_up(s, 4);
+  Cbuf.ensure((size_t)ldV * ldc);
+  for (int pass = 0; pass < p.reorth_passes; ++pass) {
+    gemm_tn<T, T, T>(cx.stream, n, mall, r, 1.0, B.V.p, ldV, W, ldw, 0.0, Cbuf.p, ldc, sw.gw.p,
+                     sw.gw.n);
+    gemm_nn<T, T, T>(cx.stream, n, mall, r, -1.0, B.V.p, ldV, Cbuf.p, ldc, false, 1.0, W, ldw);
+    gemm_tn<T, T, T>(cx.stream, n, r, r, 1.0, Qj, ldV, W, ldw, 0.0, Cbuf.p, ldc, sw.gw.p, sw.gw.n);
+    gemm_nn<T, T, T>(cx.stream, n, r, r, -1.0, Qj, ldV, Cbuf.p, ldc, false, 1.0, W, ldw);
+    gemm_flops += 4.0 * n * mall * r + 4.0 * n * r * r;
+  }
+  // [Q_{j+1}, B_j] = block_qr_factor(W; rtol)   (polfed.jl:1090)
+  T* Qn = B.V.p + (size_t)(j + 1) * s;
+  const int rn = cholqr<T>(cx, dn, sw.ow, n, r, W, ldw, Qn, ldV, tmp, ldt, p.rtol, &sw.R);
+  gemm_flops += 8.0 * n * r * r;
+  if (rn == 0) {  // rank collapse (polfed.jl:1091)
+    B.done = true;
+    return;
+  }
+  if (rn < s) fill_block<T>(cx.stream, n, s - rn, Qn + rn, ldV, T(0));
+  // B_j = R[1:rn, 1:r] (column-major); device copy readable as Bᵀ: element (k, i) at k*s + i
+  std::vector<double> Bj((size_t)rn * r), Bt((size_t)s * s, 0.0);
+  for (int c = 0; c < r; ++c)
+    for (int i = 0; i < rn; ++i) {
+      Bj[(size_t)c * rn + i] = sw.R[(size_t)c * r + i];
+      Bt[(size_t)c * s + i] = sw.R[(size_t)c * r + i];
+    }
+  SMALGO_HIP(hipMemcpyAsync(B.Bprev.p, Bt.data(), sizeof(double) * s * s, hipMemcpyHostToDevice,
+                            cx.stream));
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  B.Bblk.push_back(std::move(Bj));
+  B.r_prev = r;
+  B.r_cur = rn;
+}
+
+struct ExtractStats {
+  long long refine_steps = 0;
+  int locked = 0;
+  double gemm_flops = 0;
+};
+
+// extract_pairs (polfed.jl:892-984) + the return rule of polfed (:1148-1157)
+// on a finished basis; the mixed-precision branch adds the FP64 refinement.
+template <typename T>
+static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, const Basis<T>& Bs,
+                              int nev, int degree, double smin, double smax,
+                              const smalgo_polfed_params& p, ExtractStats& xs) {
+  const int n = op.n, s = Bs.s, ldV = Bs.ldV;
+  const std::vector<int>& ranks = Bs.ranks;
+  const auto& Ablk = Bs.Ablk;
+  const auto& Bblk = Bs.Bblk;
+  const T* Vp = Bs.V.p;
   const int nb = (int)ranks.size();
-  // ---------------- extraction (polfed.jl:892-984) ----------------
+  long long refine_steps = 0;
+  int locked_total = 0;
+  double gemm_flops = 0;
+  double t_refine = 0;
   size_t hx = tm.begin("extract");
   Result* out = new Result();
   out->n = n;
@@ -908,9 +928,6 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
   DBuf<double> Xfin;
   int ldxf = 4;
   std::vector<int> final_cols;
-  double t_refine = 0;
-  long long refine_steps = 0;
-  int locked_total = 0;
   if (M > 0) {
     std::vector<double> Th((size_t)M * M, 0.0);
     int o = 0;
@@ -947,7 +964,7 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
     gather_cols_colmajor(cx.stream, M, m, Td.p, M, dsel.p, drow.p, U.p, ldu);
     // X = V U  (build_X!, polfed.jl:767-776): one MFMA GEMM instead of 64-wide chunks
     DBuf<double> X((size_t)n * ldu);
-    gemm_nn<T, double, double>(cx.stream, n, Mpad, m, 1.0, V.p, ldV, U.p, ldu, false, 0.0, X.p, ldu);
+    gemm_nn<T, double, double>(cx.stream, n, Mpad, m, 1.0, Vp, ldV, U.p, ldu, false, 0.0, X.p, ldu);
     gemm_flops += 2.0 * n * Mpad * m;
     RayleighWork rw;
     std::vector<double> lam, res;
@@ -1177,23 +1194,83 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
     SMALGO_HIP(hipMalloc((void**)&out->Xdev, sizeof(double) * (size_t)n * kout));
     DBuf<int> dc = upload_idx(cx, cols);
     rm_to_cm<double>(cx.stream, n, kout, Xfin.p, ldxf, dc.p, out->Xdev, n);
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
   }
-  auto ph = tm.resolve();
-  const double t_total = std::chrono::duration<double>(clk::now() - t0).count();
-  t_filter = ph["filter"];
-  t_refine = ph["refine"];
-  out->stats["t_total"] = t_total;
-  out->stats["t_filter"] = t_filter;
-  out->stats["t_orth"] = ph["orth"];
-  out->stats["t_extract"] = ph["extract"];
-  out->stats["t_refine"] = t_refine;
-  out->stats["filter_steps"] = (double)filter_steps;
   out->stats["block_steps"] = nb;
   out->stats["basis_dim"] = M;
   out->stats["iterations"] = nb;
-  out->stats["refine_filter_steps"] = (double)refine_steps;
-  out->stats["locked"] = locked_total;
-  out->stats["gemm_flops"] = gemm_flops;
+  xs.refine_steps += refine_steps;
+  xs.locked += locked_total;
+  xs.gemm_flops += gemm_flops;
+  (void)t_refine;
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// POLFED (one interval)
+// ---------------------------------------------------------------------------
+template <typename T>
+static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smalgo_polfed_params& p,
+                          const double* Q1host) {
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  const int n = op.n;
+  if (nev < 1) throw Error(-1, "nev must be positive");
+  if (degree < 0) throw Error(-1, "degree must be nonnegative");
+  if (p.s < 1) throw Error(-1, "block size s must be positive");
+  if (p.s > n) throw Error(-1, "block size s must not exceed n");
+  double smin = p.smin, smax = p.smax;
+  if (!p.has_bounds) {
+    if (op.is_callback()) throw Error(-1, "Matrix-free POLFED requires spectral bounds smin and smax");
+    smin = op.A->gersh_min;
+    smax = op.A->gersh_max;
+  }
+  if (!(smax > smin)) throw Error(-1, "need smin < smax");
+  const std::vector<double> coeffs =
+      build_coeffs(degree, smin, smax, p.sigma, p.emin, p.emax, p.window_type, p.jackson != 0);
+  const int s = p.s;
+  const int mmax = p.mmax > 0 ? p.mmax : (int)std::ceil(2.8 * nev / (double)s);
+  const int lds = round_up(s, 4);
+  PhaseTimer tm(cx.stream);
+  Dense dn(cx);
+  StepWork sw;
+  Basis<T> B;
+  basis_alloc<T>(B, n, s, mmax);
+  DBuf<T> W((size_t)n * lds), b0((size_t)n * lds), b1((size_t)n * lds), Cbuf;
+  DBuf<double> pbuf;
+  if (op.is_callback()) pbuf.alloc((size_t)n * lds);
+  fill_block<T>(cx.stream, n, B.ldV, B.V.p, B.ldV, T(0));
+  basis_start<T>(cx, dn, sw.ow, B, n, Q1host, p.seed, p.rtol);
+  FilterTile basis_tile{};
+  long long filter_steps = 0;
+  double gemm_flops = 0;
+  for (int j = 0; j < mmax && !B.done; ++j) {
+    T* Qj = B.V.p + (size_t)j * s;
+    size_t hf = tm.begin("filter");
+    apply_filter<T>(cx, op, B.r_cur, Qj, B.ldV, coeffs, smin, smax, W.p, lds, b0.p, b1.p, lds,
+                    pbuf.p);
+    tm.end(hf);
+    if (j == 0) basis_tile = cx.last_tile;
+    filter_steps += degree;  // K fused steps (the all-zero first step is skipped)
+    size_t ho = tm.begin("orth");
+    const int r = B.r_cur;
+    lanczos_step<T>(cx, dn, sw, p, n, B, j, W.p, lds, b0.p, lds, Cbuf, gemm_flops);
+    (void)r;
+    tm.end(ho);
+  }
+  ExtractStats xs;
+  Result* out = polfed_extract<T>(cx, dn, tm, op, B, nev, degree, smin, smax, p, xs);
+  auto ph = tm.resolve();
+  const int nb = (int)B.ranks.size();
+  out->stats["t_total"] = std::chrono::duration<double>(clk::now() - t0).count();
+  out->stats["t_filter"] = ph["filter"];
+  out->stats["t_orth"] = ph["orth"];
+  out->stats["t_extract"] = ph["extract"];
+  out->stats["t_refine"] = ph["refine"];
+  out->stats["filter_steps"] = (double)filter_steps;
+  out->stats["refine_filter_steps"] = (double)xs.refine_steps;
+  out->stats["locked"] = xs.locked;
+  out->stats["gemm_flops"] = gemm_flops + xs.gemm_flops;
   out->stats["smin"] = smin;
   out->stats["smax"] = smax;
   out->stats["tile_vec"] = basis_tile.vec;
@@ -1204,7 +1281,7 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
   double bytes = 0;
   for (int j = 0; j < nb; ++j)
     bytes += (double)degree * ((double)(op.A ? op.A->nnz : 0) * (sizeof(T) + 4) + 4.0 * (n + 1) +
-                               4.0 * n * ranks[j] * sizeof(T));
+                               4.0 * n * B.ranks[j] * sizeof(T));
   out->stats["spmm_bytes"] = bytes;
   return out;
 }
@@ -1376,6 +1453,595 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
   std::snprintf(buf, sizeof buf, "it=%d inside=%zu", its, out->lam.size());
   out->line = buf;
   return out;
+}
+
+// ---------------------------------------------------------------------------
+// Batched spectral slices (no reference counterpart: polfed, polfed.jl:1128,
+// treats one interval per call).  The current blocks of many intervals are
+// packed column-wise into dense n x pw panels and filtered together: every
+// column carries its own Chebyshev coefficients, so one launch serves
+// several slices, and narrow slices share the gathered rows of a wide panel.
+// ---------------------------------------------------------------------------
+struct SlicePack {
+  struct Seg {
+    int slice, c0, w, panel, off;  // slice columns [c0, c0+w) -> panel column off
+  };
+  int pw = 0, npan = 0;
+  std::vector<Seg> segs;
+  std::vector<int> panel_K;                  // degree of each panel: max over its slices
+  std::vector<std::vector<int>> col_slice;   // slice of every panel column (-1: padding)
+  long long steps = 0;                       // Σ_p K_p
+};
+
+// Slices in order of decreasing degree fill panels of width pw left to right;
+// a slice that does not fit the room left continues in the next panel.
+// Sorting by degree keeps slices of similar degree together, so the columns
+// that wait through leading zero coefficients (degree below the panel's) are few.
+static SlicePack pack_slices(const std::vector<int>& widths, const std::vector<int>& degrees,
+                             int pw) {
+  SlicePack pk;
+  pk.pw = pw;
+  const int ns = (int)widths.size();
+  std::vector<int> ord(ns);
+  for (int i = 0; i < ns; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return degrees[a] > degrees[b]; });
+  int panel = 0, off = 0;
+  for (int i : ord) {
+    int c0 = 0;
+    while (c0 < widths[i]) {
+      if ((int)pk.col_slice.size() <= panel) {
+        pk.col_slice.emplace_back(pw, -1);
+        pk.panel_K.push_back(0);
+      }
+      const int w = std::min(widths[i] - c0, pw - off);
+      pk.segs.push_back({i, c0, w, panel, off});
+      for (int c = 0; c < w; ++c) pk.col_slice[panel][off + c] = i;
+      pk.panel_K[panel] = std::max(pk.panel_K[panel], degrees[i]);
+      c0 += w;
+      off += w;
+      if (off == pw) {
+        ++panel;
+        off = 0;
+      }
+    }
+  }
+  pk.npan = (int)pk.col_slice.size();
+  for (int K : pk.panel_K) pk.steps += K;
+  return pk;
+}
+
+template <typename T>
+struct SliceFilter {
+  // One fused step of a panel: out = cq∘Q + a1∘(A X1) + b1∘X1 - s2∘X2 with
+  // per-column scalars; operands by role (0 none, 1 Q, 2 P0, 3 P1, 4 Y).
+  struct Step {
+    int x1, x2, out;
+    size_t cq;                       // offset of the cq row in the table
+    size_t a1, b1;                   // offsets of the a1/b1 rows
+  };
+  Ctx& cx;
+  const Op& op;
+  int n = 0;
+  SlicePack pk;
+  FilterTile tile{};
+  std::vector<int> widths;
+  DBuf<T> Qp, Yp, P0, P1, tab;  // panels, ping-pong buffers, coefficient table
+  size_t ones = 0;              // offset of the s2 row (1 on slice columns, 0 on padding)
+  std::vector<std::vector<Step>> steps;
+  std::vector<hipGraphExec_t> execs;
+  std::vector<hipGraph_t> graphs;
+  int graph_nodes_max = 2048;  // launches per captured graph
+  double ms_per_step = 0;      // autotuned time of one panel step
+
+  SliceFilter(Ctx& c, const Op& o) : cx(c), op(o), n(o.n) {}
+  SliceFilter(const SliceFilter&) = delete;
+  SliceFilter& operator=(const SliceFilter&) = delete;
+  ~SliceFilter() { drop_graphs(); }
+
+  void drop_graphs() {
+    for (auto e : execs) (void)hipGraphExecDestroy(e);
+    for (auto g : graphs) (void)hipGraphDestroy(g);
+    execs.clear();
+    graphs.clear();
+  }
+
+  // Build the packing, the per-column coefficient table and the step list.
+  // pw_force > 0 fixes the panel width; otherwise every admissible width is
+  // packed and timed, and the one with the least predicted time is kept.
+  void plan(const std::vector<int>& w, const std::vector<const std::vector<double>*>& co,
+            double smin, double smax, int pw_force) {
+    drop_graphs();
+    widths = w;
+    std::vector<int> deg(w.size());
+    for (size_t i = 0; i < w.size(); ++i) deg[i] = (int)co[i]->size() - 1;
+    const int pfull = panel_width<T>();
+    std::vector<int> cands;
+    if (pw_force > 0) {
+      cands.push_back(pw_force);
+    } else {
+      for (int q = pfull / 4; q <= pfull; q *= 2) cands.push_back(q);
+    }
+    const int pmax = *std::max_element(cands.begin(), cands.end());
+    P0.ensure((size_t)n * pmax);
+    P1.ensure((size_t)n * pmax);
+    double best = 1e300;
+    SlicePack bestpk;
+    FilterTile besttile{};
+    double bestms = 0;
+    for (int pw : cands) {
+      SlicePack cand = pack_slices(w, deg, pw);
+      float ms = 0;
+      const FilterTile t = tune(pw, &ms);
+      const double pred = (double)cand.steps * ms;
+      if (pred < best) {
+        best = pred;
+        bestpk = std::move(cand);
+        besttile = t;
+        bestms = ms;
+      }
+    }
+    pk = std::move(bestpk);
+    tile = besttile;
+    ms_per_step = bestms;
+    build_table(co, smin, smax);
+    const size_t tot = (size_t)n * pk.pw * pk.npan;
+    Qp.ensure(tot);
+    Yp.ensure(tot);
+    fill_block<T>(cx.stream, n, pk.pw * pk.npan, Qp.p, pk.pw * pk.npan, T(0));
+    fill_block<T>(cx.stream, n, pk.pw, P0.p, pk.pw, T(0));
+    fill_block<T>(cx.stream, n, pk.pw, P1.p, pk.pw, T(0));
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  }
+
+  // Autotuner for the per-column kernel at panel width pw: time a few steps
+  // of every (vec, lanes per row) tile on the real matrix.
+  FilterTile tune(int pw, float* ms_out) {
+    const long long key = ((long long)sizeof(T) << 56) ^ ((long long)pw << 32) ^
+                          ((long long)op.mat_id << 4) ^ (1LL << 3);
+    auto it = cx.tuned.find(key);
+    if (it != cx.tuned.end() && cx.tuned_ms.count(key)) {
+      *ms_out = cx.tuned_ms[key];
+      return it->second;
+    }
+    DBuf<T> co((size_t)pw);
+    fill_block<T>(cx.stream, 1, pw, co.p, pw, T(0.25));
+    fill_block<T>(cx.stream, n, pw, P0.p, pw, T(0));
+    fill_block<T>(cx.stream, n, pw, P1.p, pw, T(0));
+    std::vector<FilterTile> cand;
+    if (!env_tile("SMALGO_SLICE_TILE", &tile)) {
+      for (int v : {1, 2, 4})
+        for (int l : {4, 8, 16, 32, 64})
+          if (l * v <= pw && pw % v == 0 && l * v * (int)sizeof(T) <= 2048)
+            cand.push_back(FilterTile{v, l, 1});
+    } else {
+      cand.push_back(tile);
+    }
+    if (!cx.autotune && cand.size() > 1) {
+      const FilterTile d = choose_tile(sizeof(T), pw, {P0.p, P1.p}, {pw});
+      cand.assign(1, FilterTile{d.vec, d.lpr, 1});
+    }
+    hipEvent_t e0, e1;
+    SMALGO_HIP(hipEventCreate(&e0));
+    SMALGO_HIP(hipEventCreate(&e1));
+    FilterTile best = cand.front();
+    float best_ms = 1e30f;
+    for (const FilterTile& t : cand) {
+      launch_clenshaw_step_cols<T>(cx.stream, *op.A, pw, P1.p, pw, co.p, P0.p, pw, co.p, co.p,
+                                   P1.p, pw, co.p, P1.p, pw, t);
+      SMALGO_HIP(hipEventRecord(e0, cx.stream));
+      for (int r = 0; r < 4; ++r)
+        launch_clenshaw_step_cols<T>(cx.stream, *op.A, pw, P1.p, pw, co.p, P0.p, pw, co.p, co.p,
+                                     P1.p, pw, co.p, P1.p, pw, t);
+      SMALGO_HIP(hipEventRecord(e1, cx.stream));
+      SMALGO_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms / 4 < best_ms) {
+        best_ms = ms / 4;
+        best = t;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cx.tuned[key] = best;
+    cx.tuned_ms[key] = best_ms;
+    *ms_out = best_ms;
+    return best;
+  }
+
+  // The Clenshaw recurrence of every panel written out as per-column scalar
+  // rows, following the same schedule as the single-block filter (the first
+  // two steps folded into the Q term, in-place B2 overwrite afterwards).
+  // A column whose slice has degree K_i below the panel's K_p sees zero
+  // coefficients for k > K_i: its state stays exactly zero until step K_i,
+  // so it computes exactly the degree-K_i recurrence of its own slice.
+  // Padding columns have all four scalars zero and stay zero.
+  void build_table(const std::vector<const std::vector<double>*>& co, double smin, double smax) {
+    const int pw = pk.pw;
+    const double c = 0.5 * (smin + smax);
+    const double invd = 2.0 / (smax - smin);
+    const double al = 2.0 * invd, be = -2.0 * c * invd;
+    std::vector<T> h;
+    auto row = [&]() {
+      h.resize(h.size() + pw, T(0));
+      return h.size() - pw;
+    };
+    ones = row();
+    steps.assign(pk.npan, {});
+    for (int p = 0; p < pk.npan; ++p) {
+      const std::vector<int>& cs = pk.col_slice[p];
+      for (int j = 0; j < pw; ++j)
+        if (cs[j] >= 0) h[ones + j] = T(1);
+      auto coef = [&](int col, int k) -> double {
+        const int s = cs[col];
+        if (s < 0) return 0.0;
+        const std::vector<double>& v = *co[s];
+        return k < (int)v.size() ? v[k] : 0.0;
+      };
+      // constant rows α, β (slice columns only) and the final-step rows
+      const size_t ra = row(), rb = row(), rfa = row(), rfb = row();
+      for (int j = 0; j < pw; ++j)
+        if (cs[j] >= 0) {
+          h[ra + j] = (T)al;
+          h[rb + j] = (T)be;
+          h[rfa + j] = (T)invd;
+          h[rfb + j] = (T)(-c * invd);
+        }
+      const int K = pk.panel_K[p];
+      // state of B1/B2: role (0 none, 1 implicit multiple of Q, 2/3 buffer)
+      // and, for implicit, the degree whose coefficient scales Q
+      int r1 = 0, r2 = 0, k1 = -1, k2 = -1;
+      auto emit = [&](int k_cq, size_t arow, size_t brow, int out) {
+        Step st{};
+        st.out = out;
+        // X1 term
+        if (r1 >= 2) {
+          st.x1 = r1;
+          st.a1 = arow;
+          st.b1 = brow;
+        } else if (r1 == 1) {
+          st.x1 = 1;
+          st.a1 = row();
+          st.b1 = row();
+          for (int j = 0; j < pw; ++j) {
+            const double q = coef(j, k1);
+            h[st.a1 + j] = (T)(q * (double)h[arow + j]);
+            h[st.b1 + j] = (T)(q * (double)h[brow + j]);
+          }
+        } else {
+          st.x1 = 0;
+          st.a1 = st.b1 = ones;  // unused without X1
+        }
+        // Q term, with an implicit B2 folded in
+        st.cq = row();
+        for (int j = 0; j < pw; ++j) {
+          double v = coef(j, k_cq);
+          if (r2 == 1) v -= coef(j, k2);
+          h[st.cq + j] = (T)v;
+        }
+        st.x2 = r2 >= 2 ? r2 : 0;
+        steps[p].push_back(st);
+      };
+      for (int k = K + 1; k >= 2; --k) {
+        if (k == K + 1) {
+          r1 = 1;
+          k1 = K;
+          r2 = 0;
+          continue;
+        }
+        const int out = r2 >= 2 ? r2 : (r1 == 2 ? 3 : 2);
+        emit(k - 1, ra, rb, out);
+        r2 = r1;
+        k2 = k1;
+        r1 = out;
+        k1 = -1;
+      }
+      emit(0, rfa, rfb, 4);
+    }
+    tab.alloc(std::max<size_t>(h.size(), 1));
+    SMALGO_HIP(hipMemcpyAsync(tab.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice,
+                              cx.stream));
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  }
+
+  // slice i's n x widths[i] block (row-major, ld) into / out of the panels
+  void load(int i, const T* src, int lds) {
+    for (const auto& sg : pk.segs)
+      if (sg.slice == i)
+        copy_block<T, T>(cx.stream, n, sg.w, src + sg.c0, lds,
+                         Qp.p + (size_t)sg.panel * n * pk.pw + sg.off, pk.pw);
+  }
+  void store(int i, T* dst, int ldd) {
+    for (const auto& sg : pk.segs)
+      if (sg.slice == i)
+        copy_block<T, T>(cx.stream, n, sg.w, Yp.p + (size_t)sg.panel * n * pk.pw + sg.off, pk.pw,
+                         dst + sg.c0, ldd);
+  }
+
+  void launch(int p, const Step& st) {
+    const int pw = pk.pw;
+    T* Qn = Qp.p + (size_t)p * n * pw;
+    T* Yn = Yp.p + (size_t)p * n * pw;
+    auto ptr = [&](int role) -> T* {
+      switch (role) {
+        case 1: return Qn;
+        case 2: return P0.p;
+        case 3: return P1.p;
+        case 4: return Yn;
+        default: return nullptr;
+      }
+    };
+    launch_clenshaw_step_cols<T>(cx.stream, *op.A, pw, Qn, pw, tab.p + st.cq, ptr(st.x1), pw,
+                                 tab.p + st.a1, tab.p + st.b1, ptr(st.x2), pw, tab.p + ones,
+                                 ptr(st.out), pw, tile);
+  }
+
+  // Y-panels = p_i(Â) Q-panels for every slice.  With graph = true the
+  // launches are captured once per plan into HIP graphs of at most
+  // graph_nodes_max kernels (kernel nodes only) and replayed afterwards.
+  void run(bool graph) {
+    if (!graph) {
+      for (int p = 0; p < pk.npan; ++p)
+        for (const Step& st : steps[p]) launch(p, st);
+      return;
+    }
+    if (execs.empty()) capture();
+    for (auto e : execs) SMALGO_HIP(hipGraphLaunch(e, cx.stream));
+  }
+
+  void capture() {
+    std::vector<std::pair<int, size_t>> all;
+    for (int p = 0; p < pk.npan; ++p)
+      for (size_t k = 0; k < steps[p].size(); ++k) all.push_back({p, k});
+    for (size_t b = 0; b < all.size(); b += graph_nodes_max) {
+      const size_t e = std::min(all.size(), b + (size_t)graph_nodes_max);
+      hipGraph_t g = nullptr;
+      SMALGO_HIP(hipStreamBeginCapture(cx.stream, hipStreamCaptureModeThreadLocal));
+      try {
+        for (size_t k = b; k < e; ++k) launch(all[k].first, steps[all[k].first][all[k].second]);
+      } catch (...) {
+        (void)hipStreamEndCapture(cx.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      SMALGO_HIP(hipStreamEndCapture(cx.stream, &g));
+      graphs.push_back(g);
+      hipGraphExec_t ex = nullptr;
+      SMALGO_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      execs.push_back(ex);
+    }
+  }
+
+  size_t launches() const {
+    size_t t = 0;
+    for (const auto& v : steps) t += v.size();
+    return t;
+  }
+  // bytes one run moves by the model of a fused step (padding included)
+  double bytes() const {
+    const double per = (double)op.A->nnz * (sizeof(T) + 4) + 4.0 * (n + 1) +
+                       4.0 * n * pk.pw * sizeof(T);
+    return per * (double)launches();
+  }
+};
+
+static int slice_pw_env() {
+  const char* e = std::getenv("SMALGO_SLICE_PW");
+  return e ? std::atoi(e) : 0;
+}
+
+template <typename T>
+static void apply_poly_slices_t(Ctx& cx, const Op& op, const std::vector<int>& widths,
+                                const std::vector<std::vector<double>>& coeffs, double smin,
+                                double smax, int pw_force, bool graph, int replays, const double* Q,
+                                int ldq, double* Y, int ldy, double* stats) {
+  const int n = op.n, ns = (int)widths.size();
+  int wtot = 0;
+  for (int w : widths) wtot += w;
+  std::vector<const std::vector<double>*> co;
+  for (const auto& v : coeffs) co.push_back(&v);
+  SliceFilter<T> sf(cx, op);
+  sf.plan(widths, co, smin, smax, pw_force);
+  const int ld = round_up(std::max(wtot, 1), 4);
+  DBuf<T> blk((size_t)n * ld);
+  hipEvent_t e0, e1;
+  SMALGO_HIP(hipEventCreate(&e0));
+  SMALGO_HIP(hipEventCreate(&e1));
+  double ms_tot = 0;
+  for (int r = 0; r < replays; ++r) {
+    // the host round trip between replays is ordinary eager work
+    upload_cm<T>(cx, n, wtot, Q + (size_t)r * ldq * wtot, ldq, blk.p, ld);
+    int off = 0;
+    for (int i = 0; i < ns; ++i) {
+      sf.load(i, blk.p + off, ld);
+      off += widths[i];
+    }
+    SMALGO_HIP(hipEventRecord(e0, cx.stream));
+    sf.run(graph);
+    SMALGO_HIP(hipEventRecord(e1, cx.stream));
+    off = 0;
+    for (int i = 0; i < ns; ++i) {
+      sf.store(i, blk.p + off, ld);
+      off += widths[i];
+    }
+    download_cm<T>(cx, n, wtot, blk.p, ld, Y + (size_t)r * ldy * wtot, ldy);
+    float ms = 0;
+    SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
+    ms_tot += ms;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (stats) {
+    stats[0] = sf.pk.pw;
+    stats[1] = sf.pk.npan;
+    stats[2] = (double)sf.launches();
+    stats[3] = replays > 0 ? ms_tot / replays : 0.0;
+    stats[4] = sf.bytes();
+    stats[5] = (double)sf.execs.size();
+  }
+}
+
+void apply_poly_slices_host(Ctx& cx, const Op& op, const std::vector<int>& widths,
+                            const std::vector<std::vector<double>>& coeffs, double smin, double smax,
+                            int precision, int pw_force, bool graph, int replays, const double* Q,
+                            int ldq, double* Y, int ldy, double* stats) {
+  if (op.is_callback()) throw Error(-1, "batched slices need a sparse matrix");
+  if (widths.size() != coeffs.size() || widths.empty()) throw Error(-1, "bad slice list");
+  for (int w : widths)
+    if (w < 1) throw Error(-1, "slice widths must be positive");
+  if (pw_force < 0 || pw_force % 4) throw Error(-1, "panel width must be a multiple of 4");
+  if (precision == SMALGO_PREC_FP32_REFINE64)
+    apply_poly_slices_t<float>(cx, op, widths, coeffs, smin, smax, pw_force, graph, replays, Q,
+                               ldq, Y, ldy, stats);
+  else
+    apply_poly_slices_t<double>(cx, op, widths, coeffs, smin, smax, pw_force, graph, replays, Q,
+                                ldq, Y, ldy, stats);
+}
+
+// ---------------------------------------------------------------------------
+// POLFED over many slices at once.  Every slice runs the block Lanczos of
+// polfed (polfed.jl:1128-1158) with its own interval, degree, block width and
+// basis; the filters of all active slices at block step j share one packed
+// launch sequence (captured in HIP graphs), then each slice does its own
+// orthogonalisation and, at the end, its own extraction and return rule.
+// ---------------------------------------------------------------------------
+template <typename T>
+static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
+                                              const std::vector<SliceSpec>& sl,
+                                              const smalgo_polfed_params& p, bool graph) {
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  const int n = op.n, ns = (int)sl.size();
+  if (op.is_callback()) throw Error(-1, "batched slices need a sparse matrix");
+  if (ns < 1) throw Error(-1, "need at least one slice");
+  double smin = p.smin, smax = p.smax;
+  if (!p.has_bounds) {
+    smin = op.A->gersh_min;
+    smax = op.A->gersh_max;
+  }
+  if (!(smax > smin)) throw Error(-1, "need smin < smax");
+  std::vector<std::vector<double>> coeffs(ns);
+  std::vector<int> mmax(ns);
+  int smax_w = 1;
+  for (int i = 0; i < ns; ++i) {
+    const SliceSpec& q = sl[i];
+    if (q.nev < 1) throw Error(-1, "nev must be positive");
+    if (q.degree < 0) throw Error(-1, "degree must be nonnegative");
+    if (q.s < 1 || q.s > n) throw Error(-1, "block size s must be in [1, n]");
+    if (!(q.emax >= q.emin)) throw Error(-1, "slice interval must have emin <= emax");
+    coeffs[i] = build_coeffs(q.degree, smin, smax, q.sigma, q.emin, q.emax, p.window_type,
+                             p.jackson != 0);
+    mmax[i] = q.mmax > 0 ? q.mmax : (int)std::ceil(2.8 * q.nev / (double)q.s);
+    smax_w = std::max(smax_w, q.s);
+  }
+  PhaseTimer tm(cx.stream);
+  Dense dn(cx);
+  StepWork sw;
+  std::vector<Basis<T>> B(ns);
+  for (int i = 0; i < ns; ++i) {
+    basis_alloc<T>(B[i], n, sl[i].s, mmax[i]);
+    fill_block<T>(cx.stream, n, B[i].ldV, B[i].V.p, B[i].ldV, T(0));
+    basis_start<T>(cx, dn, sw.ow, B[i], n, nullptr, p.seed + (uint64_t)i, p.rtol);
+  }
+  const int ldw = round_up(smax_w, 4);
+  DBuf<T> W((size_t)n * ldw), tmp((size_t)n * ldw), Cbuf;
+  SliceFilter<T> sf(cx, op);
+  const int pw_force = slice_pw_env();
+  std::vector<int> planned;  // active slices the current plan was built for
+  std::vector<long long> fsteps(ns, 0);
+  double gemm_flops = 0, filter_bytes = 0;
+  int replans = 0, max_launch_graphs = 0;
+  for (int j = 0;; ++j) {
+    std::vector<int> act;
+    for (int i = 0; i < ns; ++i)
+      if (!B[i].done && j < mmax[i]) act.push_back(i);
+    if (act.empty()) break;
+    if (act != planned) {
+      std::vector<int> w;
+      std::vector<const std::vector<double>*> co;
+      for (int i : act) {
+        w.push_back(sl[i].s);
+        co.push_back(&coeffs[i]);
+      }
+      sf.plan(w, co, smin, smax, pw_force);
+      planned = act;
+      ++replans;
+    }
+    size_t hf = tm.begin("filter");
+    for (size_t k = 0; k < act.size(); ++k) {
+      const int i = act[k];
+      sf.load((int)k, B[i].V.p + (size_t)j * sl[i].s, B[i].ldV);
+    }
+    sf.run(graph);
+    tm.end(hf);
+    max_launch_graphs = std::max(max_launch_graphs, (int)sf.execs.size());
+    filter_bytes += sf.bytes();
+    size_t ho = tm.begin("orth");
+    for (size_t k = 0; k < act.size(); ++k) {
+      const int i = act[k];
+      sf.store((int)k, W.p, round_up(sl[i].s, 4));
+      fsteps[i] += sl[i].degree;
+      lanczos_step<T>(cx, dn, sw, p, n, B[i], j, W.p, round_up(sl[i].s, 4), tmp.p,
+                      round_up(sl[i].s, 4), Cbuf, gemm_flops);
+    }
+    tm.end(ho);
+    if (p.show_progress) {
+      std::printf("slices: block %d, %zu active, panel width %d, %d panels, %zu launches, %.1f s\n",
+                  j + 1, act.size(), sf.pk.pw, sf.pk.npan, sf.launches(),
+                  std::chrono::duration<double>(clk::now() - t0).count());
+      std::fflush(stdout);
+    }
+  }
+  const int pw_used = sf.pk.pw;
+  std::vector<Result*> out(ns, nullptr);
+  try {
+    for (int i = 0; i < ns; ++i) {
+      smalgo_polfed_params pp = p;
+      pp.s = sl[i].s;
+      pp.emin = sl[i].emin;
+      pp.emax = sl[i].emax;
+      pp.sigma = sl[i].sigma;
+      pp.mmax = mmax[i];
+      pp.seed = p.seed + (uint64_t)i;
+      ExtractStats xs;
+      out[i] = polfed_extract<T>(cx, dn, tm, op, B[i], sl[i].nev, sl[i].degree, smin, smax, pp, xs);
+      out[i]->stats["filter_steps"] = (double)fsteps[i];
+      out[i]->stats["refine_filter_steps"] = (double)xs.refine_steps;
+      out[i]->stats["locked"] = xs.locked;
+      out[i]->stats["gemm_flops"] = xs.gemm_flops;
+    }
+  } catch (...) {
+    for (Result* r : out) delete r;
+    throw;
+  }
+  auto ph = tm.resolve();
+  const double ttot = std::chrono::duration<double>(clk::now() - t0).count();
+  for (int i = 0; i < ns; ++i) {
+    Result* r = out[i];
+    r->stats["t_total"] = ttot;  // the whole batch
+    r->stats["t_filter"] = ph["filter"];
+    r->stats["t_orth"] = ph["orth"];
+    r->stats["t_extract"] = ph["extract"];
+    r->stats["t_refine"] = ph["refine"];
+    r->stats["smin"] = smin;
+    r->stats["smax"] = smax;
+    r->stats["slice_panel_width"] = pw_used;
+    r->stats["slice_replans"] = replans;
+    r->stats["slice_graphs"] = max_launch_graphs;
+    r->stats["slice_filter_bytes"] = filter_bytes;
+    r->stats["slice_gemm_flops"] = gemm_flops;
+    r->stats["tile_vec"] = sf.tile.vec;
+    r->stats["tile_lpr"] = sf.tile.lpr;
+    r->stats["tile_rt"] = sf.tile.rt;
+  }
+  return out;
+}
+
+std::vector<Result*> polfed_slices_solve(Ctx& cx, const Op& op, const std::vector<SliceSpec>& sl,
+                                         const smalgo_polfed_params& p, bool graph) {
+  if (p.precision == SMALGO_PREC_FP32_REFINE64) return polfed_slices_run<float>(cx, op, sl, p, graph);
+  if (p.precision != SMALGO_PREC_FP64) throw Error(-1, "unknown precision mode");
+  return polfed_slices_run<double>(cx, op, sl, p, graph);
 }
 
 }  // namespace smalgo

@@ -48,6 +48,7 @@ struct Ctx {
   std::map<long long, FilterTile> tuned;  // (precision, b, matrix id, alignment) -> tile
   bool autotune = true;
   FilterTile last_tile{};  // tile of the most recent row-major filter
+  std::map<long long, float> tuned_ms;  // per-step time of tuned per-column tiles
 };
 
 struct Mat {
@@ -130,6 +131,22 @@ void rayleigh_host(Ctx& cx, const Op& op, int k, const double* X, int ldx, doubl
 void residuals_host(Ctx& cx, const Op& op, int k, const double* X, int ldx, const double* lam,
                     double* res);
 Result* refine_rr_host(Ctx& cx, const Op& op, int k, const double* X, int ldx);
+
+// one slice of a batched POLFED run: interval, window centre, pairs wanted,
+// block width, filter degree, basis blocks (<= 0: ceil(2.8 nev / s))
+struct SliceSpec {
+  double emin, emax, sigma;
+  int nev, s, degree, mmax;
+};
+std::vector<Result*> polfed_slices_solve(Ctx& cx, const Op& op, const std::vector<SliceSpec>& sl,
+                                         const smalgo_polfed_params& p, bool graph);
+// the packed multi-slice filter alone (host column-major blocks; replays
+// consecutive input blocks through one plan); stats[6]: panel width, panels,
+// launches per run, ms per run, modelled bytes per run, graphs
+void apply_poly_slices_host(Ctx& cx, const Op& op, const std::vector<int>& widths,
+                            const std::vector<std::vector<double>>& coeffs, double smin, double smax,
+                            int precision, int pw_force, bool graph, int replays, const double* Q,
+                            int ldq, double* Y, int ldy, double* stats);
 
 void kpm_moments(Ctx& cx, const Op& op, int nmom, int R, uint64_t seed, double smin, double smax,
                  double* mu);

@@ -255,7 +255,8 @@ def apply_poly_block(A, Q: np.ndarray, coeffs: np.ndarray, smin: float, smax: fl
 _STATS = ("t_total", "t_filter", "t_orth", "t_extract", "t_refine", "filter_steps",
           "block_steps", "basis_dim", "iterations", "spmm_bytes", "gemm_flops",
           "refine_filter_steps", "locked", "smin", "smax", "warn_interval", "tile_vec",
-          "tile_lpr", "tile_rt")
+          "tile_lpr", "tile_rt", "slice_panel_width", "slice_replans", "slice_graphs",
+          "slice_filter_bytes", "slice_gemm_flops")
 
 
 def _take_result(ctx: Context, h, want_X: bool = True):
@@ -382,6 +383,29 @@ def refine_original_rr(A, X: np.ndarray, ctx: Optional[Context] = None):
 # ---------------------------------------------------------------------------
 # POLFED (polfed.jl:1128-1186)
 # ---------------------------------------------------------------------------
+def _polfed_params(ctx, s, smin, smax, sigma, emin, emax, tol, overextract, refine, window_type,
+                   jackson, mmax, rtol, reorth_passes, extra_keep, seed, chunk, show_progress,
+                   precision, refine_guard, refine_degree, refine_maxiter, refine_tol):
+    if window_type not in L.WINDOWS:
+        raise ValueError(f"Unknown window type: {window_type}")
+    p = L.PolfedParams()
+    ctx.lib.smalgo_polfed_default_params(C.byref(p))
+    p.s = s
+    if smin is not None and smax is not None:
+        p.has_bounds, p.smin, p.smax = 1, float(smin), float(smax)
+    p.sigma, p.emin, p.emax, p.tol = sigma, emin, emax, tol
+    p.overextract, p.refine = overextract, int(refine)
+    p.window_type, p.jackson = L.WINDOWS[window_type], int(jackson)
+    p.mmax = mmax if mmax is not None else 0
+    p.rtol, p.reorth_passes, p.extra_keep = rtol, reorth_passes, extra_keep
+    p.seed, p.chunk, p.show_progress = seed, chunk, int(show_progress)
+    p.precision = L.PRECISIONS[precision]
+    p.refine_guard = -1 if refine_guard is None else refine_guard
+    p.refine_degree = -1 if refine_degree is None else refine_degree
+    p.refine_maxiter, p.refine_tol = refine_maxiter, refine_tol
+    return p
+
+
 def polfed(A, nev: int, degree: int, s: int = 8, smin=None, smax=None, sigma: float = 0.0,
            emin: float = -math.inf, emax: float = math.inf, tol: float = 1e-10,
            threaded: bool = True, overextract: int = 2, refine: bool = True,
@@ -403,25 +427,12 @@ def polfed(A, nev: int, degree: int, s: int = 8, smin=None, smax=None, sigma: fl
     ``return_stats``) with the reference's return order rule
     (``polfed.jl:1148-1157``).
     """
-    if window_type not in L.WINDOWS:
-        raise ValueError(f"Unknown window type: {window_type}")
     ctx = ctx or default_context()
     dm, mf = _as_operator(A, ctx)
-    p = L.PolfedParams()
-    ctx.lib.smalgo_polfed_default_params(C.byref(p))
-    p.s = s
-    if smin is not None and smax is not None:
-        p.has_bounds, p.smin, p.smax = 1, float(smin), float(smax)
-    p.sigma, p.emin, p.emax, p.tol = sigma, emin, emax, tol
-    p.overextract, p.refine = overextract, int(refine)
-    p.window_type, p.jackson = L.WINDOWS[window_type], int(jackson)
-    p.mmax = mmax if mmax is not None else 0
-    p.rtol, p.reorth_passes, p.extra_keep = rtol, reorth_passes, extra_keep
-    p.seed, p.chunk, p.show_progress = seed, chunk, int(show_progress)
-    p.precision = L.PRECISIONS[precision]
-    p.refine_guard = -1 if refine_guard is None else refine_guard
-    p.refine_degree = -1 if refine_degree is None else refine_degree
-    p.refine_maxiter, p.refine_tol = refine_maxiter, refine_tol
+    p = _polfed_params(ctx, s, smin, smax, sigma, emin, emax, tol, overextract, refine,
+                       window_type, jackson, mmax, rtol, reorth_passes, extra_keep, seed, chunk,
+                       show_progress, precision, refine_guard, refine_degree, refine_maxiter,
+                       refine_tol)
     n = mf.n if mf is not None else dm.n
     q = None
     if Q1 is not None:
@@ -448,6 +459,107 @@ def polfed_mf(matvec: Callable, n: int, nev: int, degree: int, matblock: Optiona
               **kwargs):
     """``polfed(matvec!, n, nev, degree; matblock!, kwargs...)`` (``polfed.jl:1183-1186``)."""
     return polfed(MatrixFreeRealSymOp(n, matvec, matblock), nev, degree, **kwargs)
+
+
+# ---------------------------------------------------------------------------
+# batched spectral slices (no reference counterpart)
+# ---------------------------------------------------------------------------
+def apply_poly_slices(A, blocks, coeffs, smin: float, smax: float, precision: str = "fp64",
+                      panel_width: int = 0, use_graph: bool = True, replays: int = 1,
+                      return_stats: bool = False, ctx: Optional[Context] = None):
+    """``Y_i = p_i(Â) Q_i`` for several slices in one packed filter.
+
+    ``blocks``: list over replays of lists over slices of ``n×b_i`` arrays
+    (or one list over slices for a single replay); ``coeffs[i]``: Chebyshev
+    coefficients of slice ``i`` (degree ``len-1``).  Every replay runs the
+    same plan (and the same HIP graphs with ``use_graph``).  Returns the same
+    nesting of filtered blocks (plus the plan stats with ``return_stats``).
+    """
+    ctx = ctx or default_context()
+    dm, mf = _as_operator(A, ctx)
+    if mf is not None:
+        raise ValueError("batched slices need a sparse matrix")
+    single = isinstance(blocks[0], np.ndarray)
+    reps = [blocks] if single else list(blocks)
+    ns = len(reps[0])
+    widths = [int(b.shape[1]) for b in reps[0]]
+    n = dm.n
+    wtot = sum(widths)
+    Q = np.zeros((n, wtot * len(reps)), order="F")
+    for r, rep in enumerate(reps):
+        if [int(b.shape[1]) for b in rep] != widths:
+            raise ValueError("every replay needs the same slice widths")
+        Q[:, r * wtot:(r + 1) * wtot] = np.hstack(rep)
+    Y = np.zeros_like(Q)
+    cos = [np.ascontiguousarray(c, dtype=np.float64) for c in coeffs]
+    if len(cos) != ns:
+        raise ValueError("one coefficient vector per slice")
+    co = np.concatenate(cos)
+    w = (C.c_int * ns)(*widths)
+    dg = (C.c_int * ns)(*[len(c) - 1 for c in cos])
+    st = np.zeros(6)
+    ctx.check(ctx.lib.smalgo_apply_poly_slices(
+        ctx.h, dm.h, ns, w, dg, _dptr(co), float(smin), float(smax), L.PRECISIONS[precision],
+        int(panel_width), int(use_graph), len(reps), _dptr(Q), n, _dptr(Y), n, _dptr(st)))
+    outs = []
+    for r in range(len(reps)):
+        off = r * wtot
+        one = []
+        for b in widths:
+            one.append(np.array(Y[:, off:off + b]))
+            off += b
+        outs.append(one)
+    res = outs[0] if single else outs
+    if return_stats:
+        keys = ("panel_width", "panels", "launches", "ms_per_run", "bytes_per_run", "graphs")
+        return res, dict(zip(keys, st.tolist()))
+    return res
+
+
+def polfed_slices(A, slices, s: int = 8, smin=None, smax=None, tol: float = 1e-10,
+                  overextract: int = 2, refine: bool = True, window_type: str = "step",
+                  jackson: bool = True, rtol: float = 1e-12, reorth_passes: int = 1,
+                  extra_keep: int = 8, seed: int = 1, precision: str = "fp64",
+                  refine_guard: Optional[int] = None, refine_degree: Optional[int] = None,
+                  refine_maxiter: int = 3, refine_tol: float = 0.5e-10, use_graph: bool = True,
+                  show_progress: bool = False, want_X: bool = True,
+                  ctx: Optional[Context] = None):
+    """POLFED over many intervals of one sparse matrix in one batched run.
+
+    ``slices``: list of dicts with ``emin, emax, nev, degree`` and optional
+    ``s`` (default ``s``), ``sigma`` (default the interval midpoint) and
+    ``mmax``.  Slice ``i`` is ``polfed(A, nev_i, degree_i; emin_i, emax_i, …)``
+    (``polfed.jl:1128-1158``) with start-block seed ``seed + i``; the filters of
+    all slices share packed launches (HIP graphs with ``use_graph``).  Returns a
+    list of ``(λ, X, res, stats)`` per slice.
+    """
+    ctx = ctx or default_context()
+    dm, mf = _as_operator(A, ctx)
+    if mf is not None:
+        raise ValueError("batched slices need a sparse matrix")
+    p = _polfed_params(ctx, s, smin, smax, 0.0, -math.inf, math.inf, tol, overextract, refine,
+                       window_type, jackson, None, rtol, reorth_passes, extra_keep, seed, 64,
+                       show_progress, precision, refine_guard, refine_degree, refine_maxiter, refine_tol)
+    ns = len(slices)
+    arr = (L.SliceParams * ns)()
+    for i, d in enumerate(slices):
+        a = arr[i]
+        a.emin, a.emax = float(d["emin"]), float(d["emax"])
+        a.sigma = float(d.get("sigma", 0.5 * (a.emin + a.emax)))
+        a.nev, a.s, a.degree = int(d["nev"]), int(d.get("s", s)), int(d["degree"])
+        a.mmax = int(d.get("mmax") or 0)
+    hs = (C.c_void_p * ns)()
+    ctx.check(ctx.lib.smalgo_polfed_slices(ctx.h, dm.h, ns, arr, C.byref(p), int(use_graph), hs))
+    out = []
+    try:
+        for i in range(ns):
+            h, hs[i] = hs[i], None
+            out.append(_take_result(ctx, C.c_void_p(h), want_X))
+    finally:
+        for i in range(ns):
+            if hs[i]:
+                ctx.lib.smalgo_result_free(C.c_void_p(hs[i]))
+    return out
 
 
 # ---------------------------------------------------------------------------

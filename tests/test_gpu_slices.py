@@ -1,0 +1,148 @@
+"""Batched spectral slices on the GPU: the packed multi-slice filter against
+the FP64 oracle filter (apply_poly_block!, polfed.jl:464-501) per slice, HIP
+graph replays against eager launches, and the batched POLFED driver against
+dense eigh and against one polfed call per interval (polfed.jl:1128-1158)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import polfed_ref as P
+from sparse_matrix_algorithms_amd.lattices import anderson3d, gaussian_block
+
+pytestmark = pytest.mark.gpu
+
+# widths and degrees chosen to split slices across panel boundaries at every
+# panel width, and to cover the K = 0 and K = 1 special cases
+WIDTHS = [5, 16, 12, 33, 3]
+DEGREES = [40, 120, 7, 0, 1]
+
+
+def _slice_coeffs(smin, smax):
+    wins = [(-1.0, -0.5), (0.0, 0.3), (-3.0, 2.0), (1.0, 1.5), (-0.2, 0.2)]
+    return [P.build_coeffs(K, smin, smax, 0.5 * (a + b), a, b, "step", True)
+            for K, (a, b) in zip(DEGREES, wins)]
+
+
+@pytest.mark.parametrize("pw", [0, 16, 32, 64])
+def test_slices_filter_fp64_vs_oracle(ctx, pw):
+    import sparse_matrix_algorithms_amd as sm
+    A = anderson3d(8, 16.5, 3)
+    n = A.shape[0]
+    smin, smax = P.gershgorin_bounds(A)
+    co = _slice_coeffs(smin, smax)
+    reps = [[gaussian_block(100 * r + i, n, w) for i, w in enumerate(WIDTHS)] for r in range(3)]
+    outs, st = sm.apply_poly_slices(A, reps, co, smin, smax, precision="fp64", panel_width=pw,
+                                    use_graph=True, return_stats=True, ctx=ctx)
+    if pw:
+        assert st["panel_width"] == pw
+    assert st["graphs"] >= 1
+    for r in range(3):  # every graph replay against the oracle
+        for i in range(len(WIDTHS)):
+            ref = P.apply_poly_block(A, reps[r][i], co[i], smin, smax)
+            scale = max(np.abs(ref).max(), 1e-300)
+            err = np.abs(outs[r][i] - ref).max() / scale
+            assert err <= 1e-12, (r, i, err)
+
+
+def test_slices_filter_graph_equals_eager(ctx):
+    import sparse_matrix_algorithms_amd as sm
+    A = anderson3d(8, 16.5, 4)
+    n = A.shape[0]
+    smin, smax = P.gershgorin_bounds(A)
+    co = _slice_coeffs(smin, smax)
+    reps = [[gaussian_block(7 + 10 * r + i, n, w) for i, w in enumerate(WIDTHS)]
+            for r in range(4)]
+    for prec in ("fp64", "fp32"):
+        g = sm.apply_poly_slices(A, reps, co, smin, smax, precision=prec, panel_width=32,
+                                 use_graph=True, ctx=ctx)
+        e = sm.apply_poly_slices(A, reps, co, smin, smax, precision=prec, panel_width=32,
+                                 use_graph=False, ctx=ctx)
+        for r in range(4):
+            for i in range(len(WIDTHS)):
+                assert np.array_equal(g[r][i], e[r][i]), (prec, r, i)
+
+
+def test_slices_filter_fp32_vs_single(ctx):
+    """FP32 packed filter against the single-slice FP32 filter and the FP64
+    oracle: FP32 rounding only (relative 1e-4 of the block's magnitude)."""
+    import sparse_matrix_algorithms_amd as sm
+    A = anderson3d(8, 16.5, 5)
+    n = A.shape[0]
+    smin, smax = P.gershgorin_bounds(A)
+    co = _slice_coeffs(smin, smax)
+    blocks = [gaussian_block(50 + i, n, w) for i, w in enumerate(WIDTHS)]
+    out = sm.apply_poly_slices(A, blocks, co, smin, smax, precision="fp32", ctx=ctx)
+    for i in range(len(WIDTHS)):
+        ref = P.apply_poly_block(A, blocks[i], co[i], smin, smax)
+        one = sm.apply_poly_block(A, blocks[i], co[i], smin, smax, precision="fp32", ctx=ctx)
+        scale = np.abs(ref).max()
+        assert np.abs(out[i] - ref).max() <= 1e-4 * scale, i
+        assert np.abs(out[i] - one).max() <= 1e-4 * scale, i
+
+
+def _count_window(ev, center, nev):
+    i0 = np.searchsorted(ev, center)
+    i1 = i0 - nev // 2
+    i2 = i1 + nev - 1
+    return 0.5 * (ev[i1 - 1] + ev[i1]), 0.5 * (ev[i2] + ev[i2 + 1])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_polfed_slices_vs_eigh_and_single(ctx, precision):
+    import sparse_matrix_algorithms_amd as sm
+    A = anderson3d(10, 16.5, 22)
+    ev = np.linalg.eigh(A.toarray())[0]
+    smin, smax = P.gershgorin_bounds(A)
+    anorm = max(abs(smin), abs(smax))
+    # (centre, nev, s, K, basis factor): parameters the FP64 oracle converges with
+    specs = [(-3.0, 12, 4, 1200, 3), (0.0, 30, 8, 500, 3), (2.5, 32, 16, 800, 3.5)]
+    slices = []
+    for c, nev, s, K, f in specs:
+        a, b = _count_window(ev, c, nev)
+        slices.append(dict(emin=a, emax=b, nev=nev, s=s, degree=K,
+                           mmax=math.ceil(f * nev / s)))
+    outs = sm.polfed_slices(A, slices, smin=smin, smax=smax, precision=precision, seed=3,
+                            ctx=ctx)
+    for i, (d, (lam, X, res, st)) in enumerate(zip(slices, outs)):
+        exact = ev[(ev >= d["emin"]) & (ev <= d["emax"])]
+        assert len(exact) == d["nev"]
+        assert len(lam) == d["nev"], (i, st["line"])
+        assert np.max(np.abs(np.sort(lam) - exact)) <= 1e-10 * anorm, i
+        assert res.max() <= 1e-10 * anorm, (i, st)
+        R = A @ X - X * lam[None, :]
+        assert np.max(np.linalg.norm(R, axis=0)) <= 2e-10 * anorm
+        assert np.max(np.abs(X.T @ X - np.eye(d["nev"]))) <= 1e-8
+        # the same interval as one polfed call with the slice's seed
+        lo, _, ro = sm.polfed(A, d["nev"], d["degree"], s=d["s"], smin=smin, smax=smax,
+                              emin=d["emin"], emax=d["emax"], sigma=0.5 * (d["emin"] + d["emax"]),
+                              mmax=d["mmax"], seed=3 + i, precision=precision, ctx=ctx,
+                              want_X=False)
+        assert np.max(np.abs(np.sort(lam) - np.sort(lo))) <= 1e-10 * anorm, i
+    assert outs[0][3]["slice_graphs"] >= 1
+
+
+def test_polfed_slices_eager_matches_graph(ctx):
+    import sparse_matrix_algorithms_amd as sm
+    A = anderson3d(8, 16.5, 9)
+    ev = np.linalg.eigh(A.toarray())[0]
+    smin, smax = P.gershgorin_bounds(A)
+    slices = []
+    for c, nev, s in [(-1.0, 10, 8), (1.0, 16, 4)]:
+        a, b = _count_window(ev, c, nev)
+        slices.append(dict(emin=a, emax=b, nev=nev, s=s, degree=400,
+                           mmax=math.ceil(3 * nev / s)))
+    g = sm.polfed_slices(A, slices, smin=smin, smax=smax, use_graph=True, ctx=ctx, want_X=False)
+    e = sm.polfed_slices(A, slices, smin=smin, smax=smax, use_graph=False, ctx=ctx, want_X=False)
+    for (lg, _, rg, _), (le, _, re_, _) in zip(g, e):
+        assert np.array_equal(lg, le)
+        assert np.array_equal(rg, re_)
+
+
+def test_polfed_slices_bad_args(ctx):
+    import sparse_matrix_algorithms_amd as sm
+    A = anderson3d(6, 16.5, 1)
+    with pytest.raises(ValueError, match="nev must be positive"):
+        sm.polfed_slices(A, [dict(emin=0.0, emax=1.0, nev=0, degree=10)], ctx=ctx)
+    with pytest.raises(ValueError, match="emin <= emax"):
+        sm.polfed_slices(A, [dict(emin=1.0, emax=0.0, nev=4, degree=10)], ctx=ctx)
