@@ -40,15 +40,27 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=("anderson", "slices", "xxz"), default="anderson",
+                    help="anderson: one interval of the L=48 Anderson model (the headline); "
+                         "slices: 16 intervals of the L=64 Anderson model in one batched run; "
+                         "xxz: ~1000 pairs at the centre of the L=20 random-field XXZ chain")
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default=None,
+                    help="slices/xxz: arithmetic of the basis build (fp32 = FP32 filter + FP64 "
+                         "refinement); default fp64")
+    ap.add_argument("--slices", type=int, default=16, help="slices: number of intervals")
+    ap.add_argument("--compare", type=int, default=0,
+                    help="slices: also run the first N slices one polfed call at a time and "
+                         "as a batch of N")
+    ap.add_argument("--eager", action="store_true", help="slices: launch without HIP graphs")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--L", type=int, default=48)
+    ap.add_argument("--L", type=int, default=None, help="lattice size (48 / 64 / 20 by workload)")
     ap.add_argument("--W", type=float, default=16.5)
-    ap.add_argument("--seed", type=int, default=48)
-    ap.add_argument("--nev", type=int, default=500)
+    ap.add_argument("--seed", type=int, default=None, help="disorder seed (default: L)")
+    ap.add_argument("--nev", type=int, default=None, help="pairs wanted (500, or 1000 for xxz)")
     ap.add_argument("--fill", type=float, default=0.96,
                     help="interval sized for fill*nev KPM eigenvalues, so nev covers the window")
-    ap.add_argument("--s", type=int, default=64)
+    ap.add_argument("--s", type=int, default=None, help="block size (64, or 128 for xxz)")
     ap.add_argument("--kappa", type=float, default=8.0)
     ap.add_argument("--mmax-factor", type=float, default=2.8)
     ap.add_argument("--kpm-moments", type=int, default=256)
@@ -88,6 +100,14 @@ def main():
     if args.profile:
         args.steps, args.warmup, args.skip_fp64, args.skip_cpu = 1, 0, True, True
         args.skip_variants = True
+    if args.workload == "slices":
+        return bench_slices(args)
+    if args.workload == "xxz":
+        return bench_xxz(args)
+    args.L = args.L or 48
+    args.seed = args.L if args.seed is None else args.seed
+    args.s = args.s or 64
+    args.nev = args.nev or 500
 
     import sparse_matrix_algorithms_amd as sm
     from sparse_matrix_algorithms_amd import kpm
@@ -206,29 +226,9 @@ def main():
 
     cpu = None
     if not args.skip_cpu:
-        from oracle import clenshaw_c
-        from sparse_matrix_algorithms_amd.lattices import gaussian_block
-        threads = min(os.cpu_count() or 1, 16)
-        Q = gaussian_block(1, n, args.s)
-        co = kpm_co = sm.build_coeffs(16, smin, smax, 0.0, emin, emax, "step")
-        clenshaw_c.apply_poly_block(A, Q, co, smin, smax, nthreads=threads)  # warm
-        steps, t_cpu = 0, 0.0
-        while t_cpu < args.cpu_seconds:
-            tt = time.perf_counter()
-            clenshaw_c.apply_poly_block(A, Q, kpm_co, smin, smax, nthreads=threads)
-            t_cpu += time.perf_counter() - tt
-            steps += len(kpm_co)
-        per_step = t_cpu / steps
-        # reference FP64 algorithm: mmax·K filter steps per solve at this block width
         steps64 = st64_steps if "fp64_plain" in detail else st["filter_steps"]
-        t_cpu_solve = per_step * steps64
-        cpu = dict(value=round(nvalid / t_cpu_solve, 4), unit="eigenpairs/s", cores=threads,
-                   kind="port",
-                   sample=(f"oracle OpenMP FP64 CSR Clenshaw filter, n={n}, b={args.s}: {steps} "
-                           f"steps in {t_cpu:.1f}s = {per_step * 1e3:.2f} ms/step, projected over "
-                           f"the {int(steps64)} filter steps of the FP64 solve (filter only, "
-                           f"no orthogonalisation or extraction: a lower bound on CPU time)"))
-        log(f"[bench] cpu: {per_step * 1e3:.2f} ms/step x {int(steps64)} -> {t_cpu_solve:.1f}s")
+        cpu = cpu_baseline(A, args.s, smin, smax, emin, emax, steps64, nvalid, args.cpu_seconds,
+                           "of the FP64 solve")
 
     out = {
         "metric": "eigenpairs/s at rel. resid <=1e-10, POLFED FP32+refine vs FP64; SpMM HBM GB/s",
@@ -252,6 +252,242 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": detail,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(A, b, smin, smax, emin, emax, steps, pairs, seconds, what):
+    """Time the oracle's OpenMP FP64 Clenshaw filter (the reference algorithm's
+    dominant cost) on a bounded sample at block width b and project it over
+    `steps` fused filter steps: eigenpairs/s of a filter-only CPU solve."""
+    import sparse_matrix_algorithms_amd as sm
+    from oracle import clenshaw_c
+    from sparse_matrix_algorithms_amd.lattices import gaussian_block
+    n = A.shape[0]
+    threads = min(os.cpu_count() or 1, 16)
+    Q = gaussian_block(1, n, b)
+    co = sm.build_coeffs(16, smin, smax, 0.5 * (emin + emax), emin, emax, "step")
+    clenshaw_c.apply_poly_block(A, Q, co, smin, smax, nthreads=threads)  # warm
+    k, t_cpu = 0, 0.0
+    while t_cpu < seconds:
+        tt = time.perf_counter()
+        clenshaw_c.apply_poly_block(A, Q, co, smin, smax, nthreads=threads)
+        t_cpu += time.perf_counter() - tt
+        k += len(co)
+    per_step = t_cpu / k
+    t_solve = per_step * steps
+    log(f"[bench] cpu: {per_step * 1e3:.2f} ms/step x {int(steps)} -> {t_solve:.1f}s")
+    return dict(value=round(pairs / t_solve, 4), unit="eigenpairs/s", cores=threads, kind="port",
+                sample=(f"oracle OpenMP FP64 CSR Clenshaw filter, n={n}, b={b}: {k} steps in "
+                        f"{t_cpu:.1f}s = {per_step * 1e3:.2f} ms/step, projected over the "
+                        f"{int(steps)} filter steps {what} (filter only, no orthogonalisation or "
+                        f"extraction: a lower bound on CPU time)"))
+
+
+def filter_roofline(st, prec, percol, bytes_key="spmm_bytes", launches=None):
+    """Achieved algorithmic bandwidth of the fused Clenshaw kernel from the
+    solver's event-timed filter phase and its modelled bytes."""
+    launches = launches if launches is not None else st["filter_steps"]
+    avg_launch = st["t_filter"] / max(launches, 1)
+    bpl = st[bytes_key] / max(launches, 1)
+    achieved = bpl / avg_launch / 1e9
+    tname = "float" if prec == "fp32" else "double"
+    kname = (f"clenshaw_kernel<{tname},{int(st['tile_vec'])},{int(st['tile_lpr'])},"
+             f"{int(st['tile_rt'])},{'true' if percol else 'false'}>")
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=kname,
+                bytes_per_launch=bpl, avg_launch_us=round(avg_launch * 1e6, 3),
+                launches=int(launches))
+
+
+def _ctx_matrix(A):
+    import sparse_matrix_algorithms_amd as sm
+    ctx = sm.default_context()
+    t = time.time()
+    dm = sm.DeviceMatrix(A, ctx)
+    return ctx, dm, time.time() - t
+
+
+# 16 uneven slice sizes (50..400 eigenpairs) assigned to centres -3.75, -3.25, ..., 3.75
+SLICE_NEV = [50, 400, 100, 300, 75, 250, 150, 350, 60, 200, 125, 275, 90, 325, 175, 225]
+
+
+def slice_width(nev):
+    """Block width that keeps about 20 block steps at mmax = 2.8 nev / s (block
+    Lanczos with fewer steps converges poorly): the power of two nearest
+    2.8 nev / 20, within [8, 64]."""
+    return int(min(64, max(8, 2 ** round(math.log2(2.8 * nev / 20)))))
+
+
+def bench_slices(args):
+    """Batched spectral slicing: one polfed_slices run over `--slices` intervals
+    of the periodic 3D Anderson model (default L=64, n=262,144).  Interval i is
+    centred at -4 + 8(i+1/2)/16 and sized by KPM to hold 0.96*nev_i eigenvalues
+    (nev_i from 50 to 400); block width 8..64 grows with nev_i; degree and
+    basis size by the plain rule.  value = eigenpairs/s of the whole batch."""
+    import sparse_matrix_algorithms_amd as sm
+    from sparse_matrix_algorithms_amd import kpm
+    from sparse_matrix_algorithms_amd.lattices import anderson3d
+    L = args.L or 64
+    seed = L if args.seed is None else args.seed
+    prec = args.precision or "fp64"
+    A = anderson3d(L, args.W, seed)
+    n = A.shape[0]
+    ctx, dm, t_up = _ctx_matrix(A)
+    smin, smax = dm.gershgorin_bounds()
+    anorm = max(abs(smin), abs(smax))
+    mu, _, _ = kpm.kpm_moments(dm, args.kpm_moments, args.kpm_probes, 7, smin, smax, ctx=ctx)
+    ns = args.slices
+    slices = []
+    for i in range(ns):
+        c = -4.0 + 8.0 * (i + 0.5) / ns
+        nev = SLICE_NEV[i % len(SLICE_NEV)]
+        a, b = kpm.interval_for_count(mu, n, args.fill * nev, smin, smax, c)
+        s = slice_width(nev)
+        K, mm = kpm.plain_parameters(n, nev, a, b, smin, smax, s, args.kappa, args.mmax_factor)
+        slices.append(dict(emin=a, emax=b, sigma=c, nev=nev, s=s, degree=K, mmax=mm))
+        log(f"[bench] slice {i}: centre {c:+.3f} [{a:.5f},{b:.5f}] nev={nev} s={s} K={K} "
+            f"mmax={mm}")
+    kw = dict(smin=smin, smax=smax, tol=1e-10 * anorm, precision=prec, ctx=ctx, want_X=False,
+              refine_guard=args.refine_guard, refine_degree=args.refine_degree,
+              refine_maxiter=args.refine_maxiter)
+
+    def batch(sls):
+        t = time.perf_counter()
+        outs = sm.polfed_slices(dm, sls, use_graph=not args.eager, show_progress=True, **kw)
+        return outs, time.perf_counter() - t
+
+    def count_valid(sls, outs):
+        per = []
+        for d, (lam, _, res, _) in zip(sls, outs):
+            per.append(valid_pairs(lam, res, d["emin"], d["emax"], anorm)[0])
+        return per
+
+    outs, dt = batch(slices)
+    per = count_valid(slices, outs)
+    total = sum(per)
+    st = outs[0][3]
+    log(f"[bench] batch of {ns}: {dt:.2f}s, {total} valid pairs, t_filter={st['t_filter']:.2f}s "
+        f"panel width {st['slice_panel_width']:.0f}")
+    detail = dict(n=n, nnz=dm.nnz, upload_seconds=round(t_up, 3), precision=prec,
+                  graphs=not args.eager, t_filter=st["t_filter"], t_orth=st["t_orth"],
+                  t_extract=st["t_extract"], t_refine=st["t_refine"],
+                  panel_width=st["slice_panel_width"], replans=st["slice_replans"],
+                  graphs_per_block=st["slice_graphs"],
+                  filter_gbs=round(st["slice_filter_bytes"] / st["t_filter"] / 1e9, 1),
+                  slices=[dict(centre=d["sigma"], interval=[d["emin"], d["emax"]], nev=d["nev"],
+                               s=d["s"], degree=d["degree"], mmax=d["mmax"], valid=v,
+                               max_rel_res=float(o[2].max() / anorm) if len(o[2]) else None)
+                          for d, v, o in zip(slices, per, outs)])
+    roof = filter_roofline(dict(st, filter_steps=st["slice_launches"]), prec, True,
+                           bytes_key="slice_filter_bytes")
+    roof["kernel"] += " (mixed panel widths; tile of the widest)"
+    if args.compare:
+        k = min(args.compare, ns)
+        sub = slices[:k]
+        t_seq = 0.0
+        seq_valid = 0
+        same = True
+        bo, bdt = batch(sub)
+        for i, d in enumerate(sub):
+            t = time.perf_counter()
+            lam, _, res = sm.polfed(dm, d["nev"], d["degree"], s=d["s"], emin=d["emin"],
+                                    emax=d["emax"], sigma=d["sigma"], mmax=d["mmax"], seed=1 + i,
+                                    **kw)
+            t_seq += time.perf_counter() - t
+            nv, ok = valid_pairs(lam, res, d["emin"], d["emax"], anorm)
+            seq_valid += nv
+            lb, _, rb, _ = bo[i]
+            _, okb = valid_pairs(lb, rb, d["emin"], d["emax"], anorm)
+            a1, a2 = np.sort(lam[ok]), np.sort(lb[okb])
+            same &= bool(len(a1) == len(a2) and (len(a1) == 0 or
+                                                  np.max(np.abs(a1 - a2)) <= 1e-10 * anorm))
+        bvalid = sum(count_valid(sub, bo))
+        detail["compare"] = dict(slices=k, batched_seconds=round(bdt, 3), batched_valid=bvalid,
+                                 sequential_seconds=round(t_seq, 3), sequential_valid=seq_valid,
+                                 same_eigenvalue_sets=same,
+                                 speedup=round(t_seq / bdt, 3))
+        log(f"[bench] compare {k}: batched {bdt:.2f}s vs one call each {t_seq:.2f}s")
+    cpu = None
+    if not args.skip_cpu:
+        colsteps = sum(o[3]["filter_steps"] * d["s"] for o, d in zip(outs, slices))
+        cpu = cpu_baseline(A, 64, smin, smax, -0.01, 0.01, colsteps / 64.0, total,
+                           args.cpu_seconds, "of all slices (their column-steps / 64)")
+    out = {
+        "metric": "eigenpairs/s at rel. resid <=1e-10, batched POLFED spectral slices",
+        "value": round(total / dt, 3), "unit": "eigenpairs/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp64" if prec == "fp64" else "fp32+fp64",
+        "data": "synthetic (seeded 3D Anderson disorder)",
+        "config": {"workload": f"batched POLFED slices, 3D Anderson L={L} W={args.W} n={n}, "
+                               f"{ns} intervals centred in [-4,4] holding 50-400 pairs each, "
+                               f"block 8-64, {prec}", "L": L, "n": n, "slices": ns,
+                   "pairs_requested": sum(d["nev"] for d in slices),
+                   "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu, "detail": detail,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_xxz(args):
+    """POLFED on the random-field XXZ chain (default L=20, Sz=0, n=184,756):
+    ~1000 pairs at the centre of mass of the spectrum tr(H)/n, interval by
+    KPM, block 128, plain parameters.  One step = one polfed solve."""
+    import sparse_matrix_algorithms_amd as sm
+    from sparse_matrix_algorithms_amd import kpm
+    from sparse_matrix_algorithms_amd.lattices import xxz_sz0
+    L = args.L or 20
+    seed = L if args.seed is None else args.seed
+    s = args.s or 128
+    nev = args.nev or 1000
+    prec = args.precision or "fp64"
+    A = xxz_sz0(L, 5.0, seed)
+    n = A.shape[0]
+    ctx, dm, t_up = _ctx_matrix(A)
+    smin, smax = dm.gershgorin_bounds()
+    anorm = max(abs(smin), abs(smax))
+    e0 = float(A.diagonal().sum() / n)
+    mu, _, _ = kpm.kpm_moments(dm, args.kpm_moments, args.kpm_probes, 7, smin, smax, ctx=ctx)
+    emin, emax = kpm.interval_for_count(mu, n, args.fill * nev, smin, smax, e0)
+    K, mm = kpm.plain_parameters(n, nev, emin, emax, smin, smax, s, args.kappa, args.mmax_factor)
+    log(f"[bench] xxz L={L} n={n} nnz={dm.nnz} centre {e0:.4f} [{emin:.5f},{emax:.5f}] "
+        f"K={K} mmax={mm}")
+    kw = dict(s=s, smin=smin, smax=smax, sigma=e0, emin=emin, emax=emax, tol=1e-10 * anorm,
+              precision=prec, ctx=ctx, return_stats=True, want_X=False, mmax=mm,
+              refine_guard=args.refine_guard, refine_degree=args.refine_degree,
+              refine_maxiter=args.refine_maxiter)
+    for w in range(args.warmup):
+        sm.polfed(dm, nev, K, **kw)
+    times = []
+    for k in range(args.steps):
+        t = time.perf_counter()
+        lam, _, res, st = sm.polfed(dm, nev, K, **kw)
+        times.append(time.perf_counter() - t)
+        log(f"[bench] step {k}: {times[-1]:.2f}s {st['line']} t_filter={st['t_filter']:.2f}")
+    dt = float(np.mean(times))
+    nv, _ = valid_pairs(lam, res, emin, emax, anorm)
+    roof = filter_roofline(st, prec, False)
+    cpu = None
+    if not args.skip_cpu:
+        cpu = cpu_baseline(A, s, smin, smax, emin, emax, st["filter_steps"], nv, args.cpu_seconds,
+                           "of the solve")
+    out = {
+        "metric": "eigenpairs/s at rel. resid <=1e-10, POLFED on the XXZ chain",
+        "value": round(nv / dt, 3), "unit": "eigenpairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp64" if prec == "fp64" else "fp32+fp64",
+        "data": "synthetic (seeded random fields)",
+        "config": {"workload": f"POLFED random-field XXZ L={L} Sz=0 n={n}, J=1 Delta=1 h=5, "
+                               f"~{nev} pairs at tr(H)/n, s={s}, {prec}", "L": L, "n": n,
+                   "nev": nev, "block": s, "degree": K, "mmax": mm,
+                   "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu,
+        "detail": dict(n=n, nnz=dm.nnz, interval=[emin, emax], centre=e0, valid=nv,
+                       pairs=int(len(lam)), t_filter=st["t_filter"], t_orth=st["t_orth"],
+                       t_extract=st["t_extract"], t_refine=st.get("t_refine", 0.0),
+                       max_rel_res=float(res.max() / anorm) if len(res) else None),
     }
     print(json.dumps(out), flush=True)
 

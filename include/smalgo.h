@@ -223,7 +223,8 @@ int smalgo_polfed(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn, v
  * start block from seed p->seed + i.  At each block step the filters of all
  * active slices share one packed launch sequence: their blocks are packed
  * column-wise into dense panels with per-column Chebyshev coefficients
- * (slices of lower degree wait through leading zero coefficients), and the
+ * (slices of lower degree wait through leading zero coefficients; slices
+ * share a panel only where the measured step times say it pays), and the
  * sequence is captured in HIP graphs once and replayed (use_graph = 1) or
  * launched eagerly (0).  SMALGO_SLICE_PW fixes the panel width. */
 typedef struct smalgo_slice {
@@ -246,9 +247,10 @@ int smalgo_polfed_slices(smalgo_ctx* ctx, const smalgo_mat* A, int nslices,
  * hold `replays` consecutive host blocks, each n x Σwidths column-major
  * (leading dimension ldq / ldy, slice i at the running column offset); block r
  * goes through the same plan (and, with use_graph, the same instantiated
- * graphs) as block 0.  panel_width 0 picks it by timing.  stats (optional,
- * 6 doubles): panel width, panels, launches per run, ms per run, modelled
- * bytes per run, graphs per run. */
+ * graphs) as block 0.  panel_width > 0 gives every panel that width (a
+ * multiple of 4); 0 sizes each panel (up to 512-byte rows) from the measured
+ * step time of each width.  stats (optional, 6 doubles): widest panel,
+ * panels, launches per run, ms per run, modelled bytes per run, graphs per run. */
 int smalgo_apply_poly_slices(smalgo_ctx* ctx, const smalgo_mat* A, int nslices, const int* widths,
                              const int* degrees, const double* coeffs, double smin, double smax,
                              int precision, int panel_width, int use_graph, int replays,

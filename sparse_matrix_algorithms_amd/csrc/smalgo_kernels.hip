@@ -137,38 +137,9 @@ __global__ __launch_bounds__(256) void clenshaw_kernel(
 #pragma unroll
     for (int e = 0; e < V; ++e) acc[r][e] = T(0);
   if (full) {
-    if (gather && RT == 1) {
-      // one row per team: unrolled by four, then an exact tail (no redundant loads)
-      const int pe = p0[0] + len[0];
-      int p = p0[0];
-      for (; p + 4 <= pe; p += 4) {
-        const int j0 = colidx[p], j1 = colidx[p + 1], j2 = colidx[p + 2], j3 = colidx[p + 3];
-        const T v0 = val[p], v1 = val[p + 1], v2 = val[p + 2], v3 = val[p + 3];
-        T x0[V], x1[V], x2[V], x3[V];
-        ldv<T, V>(X1 + (size_t)j0 * ld1, x0);
-        ldv<T, V>(X1 + (size_t)j1 * ld1, x1);
-        ldv<T, V>(X1 + (size_t)j2 * ld1, x2);
-        ldv<T, V>(X1 + (size_t)j3 * ld1, x3);
-#pragma unroll
-        for (int e = 0; e < V; ++e) {
-          acc[0][e] += v0 * x0[e];
-          acc[0][e] += v1 * x1[e];
-          acc[0][e] += v2 * x2[e];
-          acc[0][e] += v3 * x3[e];
-        }
-      }
-      for (; p < pe; ++p) {
-        const int j0 = colidx[p];
-        const T v0 = val[p];
-        T x0[V];
-        ldv<T, V>(X1 + (size_t)j0 * ld1, x0);
-#pragma unroll
-        for (int e = 0; e < V; ++e) acc[0][e] += v0 * x0[e];
-      }
-    } else if (gather) {
-      // several rows per team: all index, value and gather loads unconditional
-      // (clamped index, zero weight past the row end) so the RT*U chains of a
-      // lane overlap
+    if (gather) {
+      // all index, value and gather loads unconditional (clamped index, zero
+      // weight past the row end) so the RT*U load chains of a lane overlap
       constexpr int U = RT >= 4 ? 1 : 2;
       for (int j = 0; j < maxlen; j += U) {
         int col[U][RT];
@@ -280,10 +251,8 @@ static void clenshaw_dispatch(hipStream_t st, const DevCSR& A, int b, const T* Q
 #define SMALGO_CL_V(VV, RR) \
   SMALGO_CL(VV, 4, RR) SMALGO_CL(VV, 8, RR) SMALGO_CL(VV, 16, RR) SMALGO_CL(VV, 32, RR) SMALGO_CL(VV, 64, RR)
   SMALGO_CL_V(1, 1) SMALGO_CL_V(2, 1) SMALGO_CL_V(4, 1)
-  if constexpr (!PERCOL) {
-    SMALGO_CL_V(1, 2) SMALGO_CL_V(2, 2) SMALGO_CL_V(4, 2)
-    SMALGO_CL_V(1, 4) SMALGO_CL_V(2, 4) SMALGO_CL_V(4, 4)
-  }
+  SMALGO_CL_V(1, 2) SMALGO_CL_V(2, 2) SMALGO_CL_V(4, 2)
+  SMALGO_CL_V(1, 4) SMALGO_CL_V(2, 4) SMALGO_CL_V(4, 4)
 #undef SMALGO_CL_V
 #undef SMALGO_CL
   throw Error(-3, "unsupported filter tile vec=" + std::to_string(t.vec) +

@@ -1466,47 +1466,79 @@ struct SlicePack {
   struct Seg {
     int slice, c0, w, panel, off;  // slice columns [c0, c0+w) -> panel column off
   };
-  int pw = 0, npan = 0;
+  struct Panel {
+    int w = 0, K = 0;  // width (multiple of 4) and degree (max over its slices)
+    size_t base = 0;   // element offset of the panel's n x w slab (set by the filter)
+    FilterTile tile{};
+  };
   std::vector<Seg> segs;
-  std::vector<int> panel_K;                  // degree of each panel: max over its slices
-  std::vector<std::vector<int>> col_slice;   // slice of every panel column (-1: padding)
-  long long steps = 0;                       // Σ_p K_p
+  std::vector<Panel> pan;
+  std::vector<std::vector<int>> col_slice;  // slice of every panel column (-1: padding)
+  int wmax = 0;
+  long long steps = 0;  // Σ_p K_p
+  double pred_ms = 0;   // modelled time of one run
 };
 
-// Slices in order of decreasing degree fill panels of width pw left to right;
-// a slice that does not fit the room left continues in the next panel.
-// Sorting by degree keeps slices of similar degree together, so the columns
-// that wait through leading zero coefficients (degree below the panel's) are few.
+// Packing.  Slices are taken in order of decreasing degree.  With a fixed
+// width every panel is wfix wide and a slice that does not fit the room left
+// continues in the next panel.  Otherwise panels have their own widths (up
+// to wmax, multiples of 4): a slice joins the open panel when the measured
+// cost of widening it, K_p (t(w+s) - t(w)), is below the cost of a panel of
+// its own, K_i t(s), where t(w) is the tuned time of one step at width w.
+// Joining shares the matrix read of each step and widens the gathered rows;
+// its price is the steps the joining columns wait through leading zero
+// coefficients (K_p - K_i).
+template <typename TimeFn>
 static SlicePack pack_slices(const std::vector<int>& widths, const std::vector<int>& degrees,
-                             int pw) {
+                             int wmax, int wfix, TimeFn&& t) {
   SlicePack pk;
-  pk.pw = pw;
+  pk.wmax = wfix > 0 ? wfix : wmax;
   const int ns = (int)widths.size();
   std::vector<int> ord(ns);
   for (int i = 0; i < ns; ++i) ord[i] = i;
   std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return degrees[a] > degrees[b]; });
-  int panel = 0, off = 0;
+  int used = 0;  // columns used in the open panel (-1: none open)
+  bool open = false;
+  auto start = [&](int K) {
+    pk.pan.push_back({});
+    pk.pan.back().K = K;
+    pk.col_slice.emplace_back();
+    used = 0;
+    open = true;
+  };
   for (int i : ord) {
     int c0 = 0;
     while (c0 < widths[i]) {
-      if ((int)pk.col_slice.size() <= panel) {
-        pk.col_slice.emplace_back(pw, -1);
-        pk.panel_K.push_back(0);
+      const int rem = widths[i] - c0;
+      bool join = false;
+      if (open && used < pk.wmax) {
+        if (wfix > 0) {
+          join = true;
+        } else {
+          const int take = std::min(rem, pk.wmax - used);
+          const int Kp = pk.pan.back().K;
+          const double widen = Kp * (t(round_up(used + take, 4)) - t(round_up(used, 4)));
+          const double alone = degrees[i] * t(round_up(take, 4));
+          join = widen <= alone;
+        }
       }
-      const int w = std::min(widths[i] - c0, pw - off);
-      pk.segs.push_back({i, c0, w, panel, off});
-      for (int c = 0; c < w; ++c) pk.col_slice[panel][off + c] = i;
-      pk.panel_K[panel] = std::max(pk.panel_K[panel], degrees[i]);
+      if (!join) start(degrees[i]);
+      const int w = std::min(rem, pk.wmax - used);
+      const int panel = (int)pk.pan.size() - 1;
+      pk.segs.push_back({i, c0, w, panel, used});
+      for (int c = 0; c < w; ++c) pk.col_slice[panel].push_back(i);
+      used += w;
       c0 += w;
-      off += w;
-      if (off == pw) {
-        ++panel;
-        off = 0;
-      }
+      if (used == pk.wmax) open = false;
     }
   }
-  pk.npan = (int)pk.col_slice.size();
-  for (int K : pk.panel_K) pk.steps += K;
+  for (size_t p = 0; p < pk.pan.size(); ++p) {
+    SlicePack::Panel& P = pk.pan[p];
+    P.w = wfix > 0 ? wfix : round_up((int)pk.col_slice[p].size(), 4);
+    pk.col_slice[p].resize(P.w, -1);
+    pk.steps += P.K;
+    pk.pred_ms += P.K * t(P.w);
+  }
   return pk;
 }
 
@@ -1516,22 +1548,18 @@ struct SliceFilter {
   // per-column scalars; operands by role (0 none, 1 Q, 2 P0, 3 P1, 4 Y).
   struct Step {
     int x1, x2, out;
-    size_t cq;                       // offset of the cq row in the table
-    size_t a1, b1;                   // offsets of the a1/b1 rows
+    size_t cq, a1, b1, s2;  // table offsets of the per-column rows
   };
   Ctx& cx;
   const Op& op;
   int n = 0;
   SlicePack pk;
-  FilterTile tile{};
   std::vector<int> widths;
   DBuf<T> Qp, Yp, P0, P1, tab;  // panels, ping-pong buffers, coefficient table
-  size_t ones = 0;              // offset of the s2 row (1 on slice columns, 0 on padding)
   std::vector<std::vector<Step>> steps;
   std::vector<hipGraphExec_t> execs;
   std::vector<hipGraph_t> graphs;
   int graph_nodes_max = 2048;  // launches per captured graph
-  double ms_per_step = 0;      // autotuned time of one panel step
 
   SliceFilter(Ctx& c, const Op& o) : cx(c), op(o), n(o.n) {}
   SliceFilter(const SliceFilter&) = delete;
@@ -1546,99 +1574,88 @@ struct SliceFilter {
   }
 
   // Build the packing, the per-column coefficient table and the step list.
-  // pw_force > 0 fixes the panel width; otherwise every admissible width is
-  // packed and timed, and the one with the least predicted time is kept.
+  // wfix > 0 gives every panel that width; otherwise panels are sized by the
+  // measured step time of each width (up to 512-byte rows).
   void plan(const std::vector<int>& w, const std::vector<const std::vector<double>*>& co,
-            double smin, double smax, int pw_force) {
+            double smin, double smax, int wfix) {
     drop_graphs();
     widths = w;
     std::vector<int> deg(w.size());
     for (size_t i = 0; i < w.size(); ++i) deg[i] = (int)co[i]->size() - 1;
-    const int pfull = panel_width<T>();
-    std::vector<int> cands;
-    if (pw_force > 0) {
-      cands.push_back(pw_force);
-    } else {
-      for (int q = pfull / 4; q <= pfull; q *= 2) cands.push_back(q);
-    }
-    const int pmax = *std::max_element(cands.begin(), cands.end());
-    P0.ensure((size_t)n * pmax);
-    P1.ensure((size_t)n * pmax);
-    double best = 1e300;
-    SlicePack bestpk;
-    FilterTile besttile{};
-    double bestms = 0;
-    for (int pw : cands) {
-      SlicePack cand = pack_slices(w, deg, pw);
+    const int wmax = wfix > 0 ? wfix : panel_width<T>();
+    P0.ensure((size_t)n * wmax);
+    P1.ensure((size_t)n * wmax);
+    auto t = [&](int width) {
+      if (width <= 0) return 0.0;
       float ms = 0;
-      const FilterTile t = tune(pw, &ms);
-      const double pred = (double)cand.steps * ms;
-      if (pred < best) {
-        best = pred;
-        bestpk = std::move(cand);
-        besttile = t;
-        bestms = ms;
-      }
+      tune(width, &ms);
+      return (double)ms;
+    };
+    pk = pack_slices(w, deg, panel_width<T>(), wfix, t);
+    for (auto& P : pk.pan) {
+      float ms = 0;
+      P.tile = tune(P.w, &ms);
     }
-    pk = std::move(bestpk);
-    tile = besttile;
-    ms_per_step = bestms;
     build_table(co, smin, smax);
-    const size_t tot = (size_t)n * pk.pw * pk.npan;
+    size_t tot = 0;
+    for (auto& P : pk.pan) {  // slabs one after another
+      P.base = tot;
+      tot += (size_t)n * P.w;
+    }
     Qp.ensure(tot);
     Yp.ensure(tot);
-    fill_block<T>(cx.stream, n, pk.pw * pk.npan, Qp.p, pk.pw * pk.npan, T(0));
-    fill_block<T>(cx.stream, n, pk.pw, P0.p, pk.pw, T(0));
-    fill_block<T>(cx.stream, n, pk.pw, P1.p, pk.pw, T(0));
+    for (const auto& P : pk.pan) fill_block<T>(cx.stream, n, P.w, Qp.p + P.base, P.w, T(0));
+    fill_block<T>(cx.stream, n, wmax, P0.p, wmax, T(0));
+    fill_block<T>(cx.stream, n, wmax, P1.p, wmax, T(0));
     SMALGO_HIP(hipStreamSynchronize(cx.stream));
   }
 
-  // Autotuner for the per-column kernel at panel width pw: time a few steps
-  // of every (vec, lanes per row) tile on the real matrix.
-  FilterTile tune(int pw, float* ms_out) {
-    const long long key = ((long long)sizeof(T) << 56) ^ ((long long)pw << 32) ^
+  // Autotuner for the per-column kernel at width w: time a few steps of
+  // every (vec, lanes per row, rows per team) tile on the real matrix.
+  FilterTile tune(int w, float* ms_out) {
+    const long long key = ((long long)sizeof(T) << 56) ^ ((long long)w << 32) ^
                           ((long long)op.mat_id << 4) ^ (1LL << 3);
     auto it = cx.tuned.find(key);
     if (it != cx.tuned.end() && cx.tuned_ms.count(key)) {
       *ms_out = cx.tuned_ms[key];
       return it->second;
     }
-    DBuf<T> co((size_t)pw);
-    fill_block<T>(cx.stream, 1, pw, co.p, pw, T(0.25));
-    fill_block<T>(cx.stream, n, pw, P0.p, pw, T(0));
-    fill_block<T>(cx.stream, n, pw, P1.p, pw, T(0));
+    DBuf<T> co((size_t)w);
+    fill_block<T>(cx.stream, 1, w, co.p, w, T(0.25));
+    fill_block<T>(cx.stream, n, w, P0.p, w, T(0));
+    fill_block<T>(cx.stream, n, w, P1.p, w, T(0));
     std::vector<FilterTile> cand;
-    if (!env_tile("SMALGO_SLICE_TILE", &tile)) {
+    FilterTile forced{};
+    if (env_tile("SMALGO_SLICE_TILE", &forced) && w % forced.vec == 0) {
+      cand.push_back(forced);
+    } else if (!cx.autotune) {
+      const FilterTile d = choose_tile(sizeof(T), w, {P0.p, P1.p}, {w});
+      cand.push_back(FilterTile{d.vec, d.lpr, 1});
+    } else {
       for (int v : {1, 2, 4})
         for (int l : {4, 8, 16, 32, 64})
-          if (l * v <= pw && pw % v == 0 && l * v * (int)sizeof(T) <= 2048)
-            cand.push_back(FilterTile{v, l, 1});
-    } else {
-      cand.push_back(tile);
-    }
-    if (!cx.autotune && cand.size() > 1) {
-      const FilterTile d = choose_tile(sizeof(T), pw, {P0.p, P1.p}, {pw});
-      cand.assign(1, FilterTile{d.vec, d.lpr, 1});
+          if (w % v == 0 && l * v <= std::max(w, 4) * 2 && l * v * (int)sizeof(T) <= 2048)
+            for (int rt : {1, 2, 4}) cand.push_back(FilterTile{v, l, rt});
     }
     hipEvent_t e0, e1;
     SMALGO_HIP(hipEventCreate(&e0));
     SMALGO_HIP(hipEventCreate(&e1));
     FilterTile best = cand.front();
     float best_ms = 1e30f;
-    for (const FilterTile& t : cand) {
-      launch_clenshaw_step_cols<T>(cx.stream, *op.A, pw, P1.p, pw, co.p, P0.p, pw, co.p, co.p,
-                                   P1.p, pw, co.p, P1.p, pw, t);
+    for (const FilterTile& tl : cand) {
+      launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, P1.p, w, co.p, P0.p, w, co.p, co.p, P1.p,
+                                   w, co.p, P1.p, w, tl);
       SMALGO_HIP(hipEventRecord(e0, cx.stream));
       for (int r = 0; r < 4; ++r)
-        launch_clenshaw_step_cols<T>(cx.stream, *op.A, pw, P1.p, pw, co.p, P0.p, pw, co.p, co.p,
-                                     P1.p, pw, co.p, P1.p, pw, t);
+        launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, P1.p, w, co.p, P0.p, w, co.p, co.p,
+                                     P1.p, w, co.p, P1.p, w, tl);
       SMALGO_HIP(hipEventRecord(e1, cx.stream));
       SMALGO_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
       SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
       if (ms / 4 < best_ms) {
         best_ms = ms / 4;
-        best = t;
+        best = tl;
       }
     }
     (void)hipEventDestroy(e0);
@@ -1657,44 +1674,42 @@ struct SliceFilter {
   // so it computes exactly the degree-K_i recurrence of its own slice.
   // Padding columns have all four scalars zero and stay zero.
   void build_table(const std::vector<const std::vector<double>*>& co, double smin, double smax) {
-    const int pw = pk.pw;
     const double c = 0.5 * (smin + smax);
     const double invd = 2.0 / (smax - smin);
     const double al = 2.0 * invd, be = -2.0 * c * invd;
     std::vector<T> h;
-    auto row = [&]() {
-      h.resize(h.size() + pw, T(0));
-      return h.size() - pw;
-    };
-    ones = row();
-    steps.assign(pk.npan, {});
-    for (int p = 0; p < pk.npan; ++p) {
+    steps.assign(pk.pan.size(), {});
+    for (size_t p = 0; p < pk.pan.size(); ++p) {
+      const int pw = pk.pan[p].w;
+      auto row = [&]() {
+        h.resize(h.size() + pw, T(0));
+        return h.size() - pw;
+      };
       const std::vector<int>& cs = pk.col_slice[p];
-      for (int j = 0; j < pw; ++j)
-        if (cs[j] >= 0) h[ones + j] = T(1);
       auto coef = [&](int col, int k) -> double {
-        const int s = cs[col];
-        if (s < 0) return 0.0;
-        const std::vector<double>& v = *co[s];
+        const int sl = cs[col];
+        if (sl < 0) return 0.0;
+        const std::vector<double>& v = *co[sl];
         return k < (int)v.size() ? v[k] : 0.0;
       };
-      // constant rows α, β (slice columns only) and the final-step rows
-      const size_t ra = row(), rb = row(), rfa = row(), rfb = row();
+      // constant rows: ones (s2), α, β and the final-step scalars, on slice columns only
+      const size_t r1s = row(), ra = row(), rb = row(), rfa = row(), rfb = row();
       for (int j = 0; j < pw; ++j)
         if (cs[j] >= 0) {
+          h[r1s + j] = T(1);
           h[ra + j] = (T)al;
           h[rb + j] = (T)be;
           h[rfa + j] = (T)invd;
           h[rfb + j] = (T)(-c * invd);
         }
-      const int K = pk.panel_K[p];
+      const int K = pk.pan[p].K;
       // state of B1/B2: role (0 none, 1 implicit multiple of Q, 2/3 buffer)
       // and, for implicit, the degree whose coefficient scales Q
       int r1 = 0, r2 = 0, k1 = -1, k2 = -1;
       auto emit = [&](int k_cq, size_t arow, size_t brow, int out) {
         Step st{};
         st.out = out;
-        // X1 term
+        st.s2 = r1s;
         if (r1 >= 2) {
           st.x1 = r1;
           st.a1 = arow;
@@ -1710,7 +1725,7 @@ struct SliceFilter {
           }
         } else {
           st.x1 = 0;
-          st.a1 = st.b1 = ones;  // unused without X1
+          st.a1 = st.b1 = r1s;  // unused without X1
         }
         // Q term, with an implicit B2 folded in
         st.cq = row();
@@ -1747,21 +1762,24 @@ struct SliceFilter {
   // slice i's n x widths[i] block (row-major, ld) into / out of the panels
   void load(int i, const T* src, int lds) {
     for (const auto& sg : pk.segs)
-      if (sg.slice == i)
-        copy_block<T, T>(cx.stream, n, sg.w, src + sg.c0, lds,
-                         Qp.p + (size_t)sg.panel * n * pk.pw + sg.off, pk.pw);
+      if (sg.slice == i) {
+        const SlicePack::Panel& P = pk.pan[sg.panel];
+        copy_block<T, T>(cx.stream, n, sg.w, src + sg.c0, lds, Qp.p + P.base + sg.off, P.w);
+      }
   }
   void store(int i, T* dst, int ldd) {
     for (const auto& sg : pk.segs)
-      if (sg.slice == i)
-        copy_block<T, T>(cx.stream, n, sg.w, Yp.p + (size_t)sg.panel * n * pk.pw + sg.off, pk.pw,
-                         dst + sg.c0, ldd);
+      if (sg.slice == i) {
+        const SlicePack::Panel& P = pk.pan[sg.panel];
+        copy_block<T, T>(cx.stream, n, sg.w, Yp.p + P.base + sg.off, P.w, dst + sg.c0, ldd);
+      }
   }
 
   void launch(int p, const Step& st) {
-    const int pw = pk.pw;
-    T* Qn = Qp.p + (size_t)p * n * pw;
-    T* Yn = Yp.p + (size_t)p * n * pw;
+    const SlicePack::Panel& P = pk.pan[p];
+    const int pw = P.w;
+    T* Qn = Qp.p + P.base;
+    T* Yn = Yp.p + P.base;
     auto ptr = [&](int role) -> T* {
       switch (role) {
         case 1: return Qn;
@@ -1772,8 +1790,8 @@ struct SliceFilter {
       }
     };
     launch_clenshaw_step_cols<T>(cx.stream, *op.A, pw, Qn, pw, tab.p + st.cq, ptr(st.x1), pw,
-                                 tab.p + st.a1, tab.p + st.b1, ptr(st.x2), pw, tab.p + ones,
-                                 ptr(st.out), pw, tile);
+                                 tab.p + st.a1, tab.p + st.b1, ptr(st.x2), pw, tab.p + st.s2,
+                                 ptr(st.out), pw, P.tile);
   }
 
   // Y-panels = p_i(Â) Q-panels for every slice.  With graph = true the
@@ -1781,8 +1799,8 @@ struct SliceFilter {
   // graph_nodes_max kernels (kernel nodes only) and replayed afterwards.
   void run(bool graph) {
     if (!graph) {
-      for (int p = 0; p < pk.npan; ++p)
-        for (const Step& st : steps[p]) launch(p, st);
+      for (size_t p = 0; p < pk.pan.size(); ++p)
+        for (const Step& st : steps[p]) launch((int)p, st);
       return;
     }
     if (execs.empty()) capture();
@@ -1791,8 +1809,8 @@ struct SliceFilter {
 
   void capture() {
     std::vector<std::pair<int, size_t>> all;
-    for (int p = 0; p < pk.npan; ++p)
-      for (size_t k = 0; k < steps[p].size(); ++k) all.push_back({p, k});
+    for (size_t p = 0; p < pk.pan.size(); ++p)
+      for (size_t k = 0; k < steps[p].size(); ++k) all.push_back({(int)p, k});
     for (size_t b = 0; b < all.size(); b += graph_nodes_max) {
       const size_t e = std::min(all.size(), b + (size_t)graph_nodes_max);
       hipGraph_t g = nullptr;
@@ -1819,9 +1837,16 @@ struct SliceFilter {
   }
   // bytes one run moves by the model of a fused step (padding included)
   double bytes() const {
-    const double per = (double)op.A->nnz * (sizeof(T) + 4) + 4.0 * (n + 1) +
-                       4.0 * n * pk.pw * sizeof(T);
-    return per * (double)launches();
+    double b = 0;
+    for (size_t p = 0; p < pk.pan.size(); ++p)
+      b += (double)steps[p].size() * ((double)op.A->nnz * (sizeof(T) + 4) + 4.0 * (n + 1) +
+                                      4.0 * n * pk.pan[p].w * sizeof(T));
+    return b;
+  }
+  int max_width() const {
+    int w = 0;
+    for (const auto& P : pk.pan) w = std::max(w, P.w);
+    return w;
   }
 };
 
@@ -1872,8 +1897,8 @@ static void apply_poly_slices_t(Ctx& cx, const Op& op, const std::vector<int>& w
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (stats) {
-    stats[0] = sf.pk.pw;
-    stats[1] = sf.pk.npan;
+    stats[0] = sf.max_width();
+    stats[1] = (double)sf.pk.pan.size();
     stats[2] = (double)sf.launches();
     stats[3] = replays > 0 ? ms_tot / replays : 0.0;
     stats[4] = sf.bytes();
@@ -1949,7 +1974,7 @@ static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
   const int pw_force = slice_pw_env();
   std::vector<int> planned;  // active slices the current plan was built for
   std::vector<long long> fsteps(ns, 0);
-  double gemm_flops = 0, filter_bytes = 0;
+  double gemm_flops = 0, filter_bytes = 0, filter_launches = 0;
   int replans = 0, max_launch_graphs = 0;
   for (int j = 0;; ++j) {
     std::vector<int> act;
@@ -1976,6 +2001,7 @@ static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
     tm.end(hf);
     max_launch_graphs = std::max(max_launch_graphs, (int)sf.execs.size());
     filter_bytes += sf.bytes();
+    filter_launches += (double)sf.launches();
     size_t ho = tm.begin("orth");
     for (size_t k = 0; k < act.size(); ++k) {
       const int i = act[k];
@@ -1986,13 +2012,13 @@ static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
     }
     tm.end(ho);
     if (p.show_progress) {
-      std::printf("slices: block %d, %zu active, panel width %d, %d panels, %zu launches, %.1f s\n",
-                  j + 1, act.size(), sf.pk.pw, sf.pk.npan, sf.launches(),
+      std::fprintf(stderr, "slices: block %d, %zu active, panel width %d, %d panels, %zu launches, %.1f s\n",
+                  j + 1, act.size(), sf.max_width(), (int)sf.pk.pan.size(), sf.launches(),
                   std::chrono::duration<double>(clk::now() - t0).count());
-      std::fflush(stdout);
+      std::fflush(stderr);
     }
   }
-  const int pw_used = sf.pk.pw;
+  const int pw_used = sf.max_width();
   std::vector<Result*> out(ns, nullptr);
   try {
     for (int i = 0; i < ns; ++i) {
@@ -2029,10 +2055,19 @@ static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
     r->stats["slice_replans"] = replans;
     r->stats["slice_graphs"] = max_launch_graphs;
     r->stats["slice_filter_bytes"] = filter_bytes;
+    r->stats["slice_launches"] = filter_launches;
     r->stats["slice_gemm_flops"] = gemm_flops;
-    r->stats["tile_vec"] = sf.tile.vec;
-    r->stats["tile_lpr"] = sf.tile.lpr;
-    r->stats["tile_rt"] = sf.tile.rt;
+    // tile of the widest panel of the last plan
+    FilterTile wt{};
+    int ww = -1;
+    for (const auto& P : sf.pk.pan)
+      if (P.w > ww) {
+        ww = P.w;
+        wt = P.tile;
+      }
+    r->stats["tile_vec"] = wt.vec;
+    r->stats["tile_lpr"] = wt.lpr;
+    r->stats["tile_rt"] = wt.rt;
   }
   return out;
 }

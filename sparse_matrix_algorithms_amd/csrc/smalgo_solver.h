@@ -141,7 +141,7 @@ struct SliceSpec {
 std::vector<Result*> polfed_slices_solve(Ctx& cx, const Op& op, const std::vector<SliceSpec>& sl,
                                          const smalgo_polfed_params& p, bool graph);
 // the packed multi-slice filter alone (host column-major blocks; replays
-// consecutive input blocks through one plan); stats[6]: panel width, panels,
+// consecutive input blocks through one plan); stats[6]: widest panel, panels,
 // launches per run, ms per run, modelled bytes per run, graphs
 void apply_poly_slices_host(Ctx& cx, const Op& op, const std::vector<int>& widths,
                             const std::vector<std::vector<double>>& coeffs, double smin, double smax,
