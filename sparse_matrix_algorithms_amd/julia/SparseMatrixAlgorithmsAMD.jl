@@ -21,7 +21,7 @@ using LinearAlgebra
 using Random
 using SparseArrays
 
-export polfed, chebyshev_poly_real_symm_sparse, MatrixFreeRealSymOp, gershgorin_bounds
+export polfed, polfed_slices, chebyshev_poly_real_symm_sparse, MatrixFreeRealSymOp, gershgorin_bounds
 
 const libsmalgo = get(ENV, "SMALGO_LIB", joinpath(@__DIR__, "..", "libsmalgo.so"))
 
@@ -35,6 +35,11 @@ mutable struct PolfedParams
     extra_keep::Cint; seed::UInt64; chunk::Cint; show_progress::Cint; precision::Cint
     refine_guard::Cint; refine_degree::Cint; refine_maxiter::Cint; refine_tol::Cdouble
     PolfedParams() = new()
+end
+
+# mirrors smalgo_slice (batched slices; no reference counterpart)
+struct SmalgoSlice
+    emin::Cdouble; emax::Cdouble; sigma::Cdouble; nev::Cint; s::Cint; degree::Cint; mmax::Cint
 end
 
 # mirrors smalgo_cheb_params
@@ -186,6 +191,42 @@ function polfed(matvec!::F, n::Int, nev::Int, degree::Int; matblock!::Union{Noth
                 kwargs...) where {F,G}
     Aop = isnothing(matblock!) ? MatrixFreeRealSymOp(n, matvec!) : MatrixFreeRealSymOp(n, matvec!, matblock!)
     return polfed(Aop, nev, degree; kwargs...)
+end
+
+"""
+    polfed_slices(A, slices; tol, precision, use_graph, kwargs...)
+
+Many `polfed` intervals of one sparse matrix in one batched device run.
+`slices` is a vector of named tuples `(emin, emax, nev, degree)` with optional
+`s`, `sigma` and `mmax`; slice `i` uses start-block seed `seed + i - 1`.
+Returns a vector of `(λ, X, res)`, one per slice.
+"""
+function polfed_slices(A::SparseMatrixCSC{Float64,Int}, slices; s::Int=8, smin=nothing,
+                       smax=nothing, tol::Float64=1e-10, overextract::Int=2, refine::Bool=true,
+                       window_type::Symbol=:step, jackson::Bool=true, rtol::Float64=1e-12,
+                       reorth_passes::Int=1, extra_keep::Int=8, seed::Int=1,
+                       precision::Symbol=:fp64, use_graph::Bool=true)
+    haskey(WINDOWS, window_type) || error("Unknown window type: $window_type")
+    dev = DeviceMatrix(A)
+    p = PolfedParams()
+    ccall((:smalgo_polfed_default_params, libsmalgo), Cvoid, (Ref{PolfedParams},), p)
+    p.s = s
+    if smin !== nothing && smax !== nothing
+        p.has_bounds = 1; p.smin = smin; p.smax = smax
+    end
+    p.tol = tol; p.overextract = overextract; p.refine = refine
+    p.window_type = WINDOWS[window_type]; p.jackson = jackson; p.rtol = rtol
+    p.reorth_passes = reorth_passes; p.extra_keep = extra_keep; p.seed = UInt64(seed)
+    p.show_progress = 0; p.precision = precision === :fp64 ? 0 : 1
+    sl = [SmalgoSlice(d.emin, d.emax, get(d, :sigma, (d.emin + d.emax) / 2), d.nev,
+                      get(d, :s, s), d.degree, something(get(d, :mmax, nothing), 0)) for d in slices]
+    hs = fill(C_NULL, length(sl))
+    rc = ccall((:smalgo_polfed_slices, libsmalgo), Cint,
+               (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{SmalgoSlice}, Ref{PolfedParams}, Cint,
+                Ptr{Ptr{Cvoid}}),
+               context().h, dev.h, length(sl), sl, p, use_graph, hs)
+    check(rc)
+    return [take_result(h) for h in hs]
 end
 
 # ---- Chebyshev subspace iteration -------------------------------------------------
