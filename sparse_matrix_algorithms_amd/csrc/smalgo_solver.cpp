@@ -766,6 +766,387 @@ void kpm_moments(Ctx& cx, const Op& op, int nmom, int R, uint64_t seed, double s
 }
 
 // ---------------------------------------------------------------------------
+// Batched spectral slices (no reference counterpart: polfed, polfed.jl:1128,
+// treats one interval per call).  The current blocks of many intervals are
+// packed column-wise into dense n x pw panels and filtered together: every
+// column carries its own Chebyshev coefficients, so one launch serves
+// several slices, and narrow slices share the gathered rows of a wide panel.
+// ---------------------------------------------------------------------------
+struct SlicePack {
+  struct Seg {
+    int slice, c0, w, panel, off;  // slice columns [c0, c0+w) -> panel column off
+  };
+  struct Panel {
+    int w = 0, K = 0;  // width (multiple of 4) and degree (max over its slices)
+    size_t base = 0;   // element offset of the panel's n x w slab (set by the filter)
+    FilterTile tile{};
+  };
+  std::vector<Seg> segs;
+  std::vector<Panel> pan;
+  std::vector<std::vector<int>> col_slice;  // slice of every panel column (-1: padding)
+  std::vector<std::vector<int>> col_idx;    // its column within the slice
+  int wmax = 0;
+  long long steps = 0;  // Σ_p K_p
+  double pred_ms = 0;   // modelled time of one run
+};
+
+// Packing.  Slices are taken in order of decreasing degree.  With a fixed
+// width every panel is wfix wide and a slice that does not fit the room left
+// continues in the next panel.  Otherwise panels have their own widths (up
+// to wmax, multiples of 4): a slice joins the open panel when the measured
+// cost of widening it, K_p (t(w+s) - t(w)), is below the cost of a panel of
+// its own, K_i t(s), where t(w) is the tuned time of one step at width w.
+// Joining shares the matrix read of each step and widens the gathered rows;
+// its price is the steps the joining columns wait through leading zero
+// coefficients (K_p - K_i).
+template <typename TimeFn>
+static SlicePack pack_slices(const std::vector<int>& widths, const std::vector<int>& degrees,
+                             int wmax, int wfix, TimeFn&& t) {
+  SlicePack pk;
+  pk.wmax = wfix > 0 ? wfix : wmax;
+  const int ns = (int)widths.size();
+  std::vector<int> ord(ns);
+  for (int i = 0; i < ns; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return degrees[a] > degrees[b]; });
+  int used = 0;  // columns used in the open panel (-1: none open)
+  bool open = false;
+  auto start = [&](int K) {
+    pk.pan.push_back({});
+    pk.pan.back().K = K;
+    pk.col_slice.emplace_back();
+    pk.col_idx.emplace_back();
+    used = 0;
+    open = true;
+  };
+  for (int i : ord) {
+    int c0 = 0;
+    while (c0 < widths[i]) {
+      const int rem = widths[i] - c0;
+      bool join = false;
+      if (open && used < pk.wmax) {
+        if (wfix > 0) {
+          join = true;
+        } else {
+          const int take = std::min(rem, pk.wmax - used);
+          const int Kp = pk.pan.back().K;
+          const double widen = Kp * (t(round_up(used + take, 4)) - t(round_up(used, 4)));
+          const double alone = degrees[i] * t(round_up(take, 4));
+          join = widen <= alone;
+        }
+      }
+      if (!join) start(degrees[i]);
+      const int w = std::min(rem, pk.wmax - used);
+      const int panel = (int)pk.pan.size() - 1;
+      pk.segs.push_back({i, c0, w, panel, used});
+      for (int c = 0; c < w; ++c) {
+        pk.col_slice[panel].push_back(i);
+        pk.col_idx[panel].push_back(c0 + c);
+      }
+      used += w;
+      c0 += w;
+      if (used == pk.wmax) open = false;
+    }
+  }
+  for (size_t p = 0; p < pk.pan.size(); ++p) {
+    SlicePack::Panel& P = pk.pan[p];
+    P.w = wfix > 0 ? wfix : round_up((int)pk.col_slice[p].size(), 4);
+    pk.col_slice[p].resize(P.w, -1);
+    pk.col_idx[p].resize(P.w, 0);
+    pk.steps += P.K;
+    pk.pred_ms += P.K * t(P.w);
+  }
+  return pk;
+}
+
+// Chebyshev coefficients of the columns of one slice: one vector shared by
+// all its columns, or one vector per column.  Coefficients past a vector's
+// end are zero.
+struct SliceCoeffs {
+  const std::vector<double>* shared = nullptr;
+  const std::vector<std::vector<double>>* percol = nullptr;
+  int degree() const {
+    if (shared) return (int)shared->size() - 1;
+    int K = 0;
+    for (const auto& v : *percol) K = std::max(K, (int)v.size() - 1);
+    return K;
+  }
+  double at(int j, int k) const {
+    const std::vector<double>& v = shared ? *shared : (*percol)[j];
+    return k < (int)v.size() ? v[k] : 0.0;
+  }
+};
+
+template <typename T>
+struct SliceFilter {
+  // One fused step of a panel: out = cq∘Q + a1 (A X1) + b1 X1 - s2 X2 with a
+  // per-column cq row; operands by role (0 none, 1 Q, 2 P0, 3 P1, 4 Y).
+  struct Step {
+    int x1, x2, out;
+    size_t cq;  // table offset of the cq row
+    double a1, b1, s2;
+  };
+  Ctx& cx;
+  const Op& op;
+  int n = 0;
+  SlicePack pk;
+  std::vector<int> widths;
+  DBuf<T> Qp, Yp, P0, P1, tab;  // panels, ping-pong buffers, coefficient table
+  std::vector<std::vector<Step>> steps;
+  std::vector<hipGraphExec_t> execs;
+  std::vector<hipGraph_t> graphs;
+  int graph_nodes_max = 2048;  // launches per captured graph
+
+  SliceFilter(Ctx& c, const Op& o) : cx(c), op(o), n(o.n) {}
+  SliceFilter(const SliceFilter&) = delete;
+  SliceFilter& operator=(const SliceFilter&) = delete;
+  ~SliceFilter() { drop_graphs(); }
+
+  void drop_graphs() {
+    for (auto e : execs) (void)hipGraphExecDestroy(e);
+    for (auto g : graphs) (void)hipGraphDestroy(g);
+    execs.clear();
+    graphs.clear();
+  }
+
+  // Build the packing, the per-column coefficient table and the step list.
+  // wfix > 0 gives every panel that width; otherwise panels are sized by the
+  // measured step time of each width (up to 512-byte rows).
+  void plan(const std::vector<int>& w, const std::vector<SliceCoeffs>& co, double smin,
+            double smax, int wfix) {
+    drop_graphs();
+    widths = w;
+    std::vector<int> deg(w.size());
+    for (size_t i = 0; i < w.size(); ++i) deg[i] = co[i].degree();
+    const int wmax = wfix > 0 ? wfix : panel_width<T>();
+    P0.ensure((size_t)n * wmax);
+    P1.ensure((size_t)n * wmax);
+    auto t = [&](int width) {
+      if (width <= 0) return 0.0;
+      float ms = 0;
+      tune(width, &ms);
+      return (double)ms;
+    };
+    pk = pack_slices(w, deg, panel_width<T>(), wfix, t);
+    for (auto& P : pk.pan) {
+      float ms = 0;
+      P.tile = tune(P.w, &ms);
+    }
+    build_table(co, smin, smax);
+    size_t tot = 0;
+    for (auto& P : pk.pan) {  // slabs one after another
+      P.base = tot;
+      tot += (size_t)n * P.w;
+    }
+    Qp.ensure(tot);
+    Yp.ensure(tot);
+    for (const auto& P : pk.pan) fill_block<T>(cx.stream, n, P.w, Qp.p + P.base, P.w, T(0));
+    fill_block<T>(cx.stream, n, wmax, P0.p, wmax, T(0));
+    fill_block<T>(cx.stream, n, wmax, P1.p, wmax, T(0));
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  }
+
+  // Autotuner for the per-column kernel at width w: time a few steps of
+  // every (vec, lanes per row, rows per team) tile on the real matrix.
+  FilterTile tune(int w, float* ms_out) {
+    const long long key = ((long long)sizeof(T) << 56) ^ ((long long)w << 32) ^
+                          ((long long)op.mat_id << 4) ^ (1LL << 3);
+    auto it = cx.tuned.find(key);
+    if (it != cx.tuned.end() && cx.tuned_ms.count(key)) {
+      *ms_out = cx.tuned_ms[key];
+      return it->second;
+    }
+    DBuf<T> co((size_t)w);
+    fill_block<T>(cx.stream, 1, w, co.p, w, T(0.25));
+    fill_block<T>(cx.stream, n, w, P0.p, w, T(0));
+    fill_block<T>(cx.stream, n, w, P1.p, w, T(0));
+    std::vector<FilterTile> cand;
+    FilterTile forced{};
+    if (env_tile("SMALGO_SLICE_TILE", &forced) && w % forced.vec == 0) {
+      cand.push_back(forced);
+    } else if (!cx.autotune) {
+      const FilterTile d = choose_tile(sizeof(T), w, {P0.p, P1.p}, {w});
+      cand.push_back(FilterTile{d.vec, d.lpr, 1});
+    } else {
+      for (int v : {1, 2, 4})
+        for (int l : {4, 8, 16, 32, 64})
+          if (w % v == 0 && l * v <= std::max(w, 4) * 2 && l * v * (int)sizeof(T) <= 2048)
+            for (int rt : {1, 2, 4}) cand.push_back(FilterTile{v, l, rt});
+    }
+    hipEvent_t e0, e1;
+    SMALGO_HIP(hipEventCreate(&e0));
+    SMALGO_HIP(hipEventCreate(&e1));
+    FilterTile best = cand.front();
+    float best_ms = 1e30f;
+    for (const FilterTile& tl : cand) {
+      // the general step out = cq∘Q + a1 A X1 + b1 X1 - X2, in place on X2
+      launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, P1.p, w, co.p, P0.p, w, T(0.1), T(0.1),
+                                   P1.p, w, T(1), P1.p, w, tl);
+      SMALGO_HIP(hipEventRecord(e0, cx.stream));
+      for (int r = 0; r < 4; ++r)
+        launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, P1.p, w, co.p, P0.p, w, T(0.1), T(0.1),
+                                     P1.p, w, T(1), P1.p, w, tl);
+      SMALGO_HIP(hipEventRecord(e1, cx.stream));
+      SMALGO_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms / 4 < best_ms) {
+        best_ms = ms / 4;
+        best = tl;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cx.tuned[key] = best;
+    cx.tuned_ms[key] = best_ms;
+    *ms_out = best_ms;
+    return best;
+  }
+
+  // The Clenshaw recurrence of apply_poly_block! (polfed.jl:464-501) for
+  // every panel, unfolded: B_{K+1} = c_K Q is materialised by a Q-only step
+  // (no gather), then TMP = c_{k-1} Q + α A B1 + β B1 - B2 for k = K..2 with
+  // TMP overwriting B2 in place once B2 exists, then the final
+  // Y = c_0 Q + invd A B1 - c invd B1 - B2.  Only the Q coefficient differs
+  // between columns (all slices share smin/smax), so one row of cq per step.
+  // A column whose slice has degree K_i below the panel's K_p gets zero
+  // coefficients for k > K_i: its state stays exactly zero until step K_i,
+  // so it runs exactly the degree-K_i recurrence of its own slice.  Padding
+  // columns have Q = 0 and cq = 0 and stay zero.
+  void build_table(const std::vector<SliceCoeffs>& co, double smin, double smax) {
+    const double c = 0.5 * (smin + smax);
+    const double invd = 2.0 / (smax - smin);
+    const double al = 2.0 * invd, be = -2.0 * c * invd;
+    std::vector<T> h;
+    steps.assign(pk.pan.size(), {});
+    for (size_t p = 0; p < pk.pan.size(); ++p) {
+      const int pw = pk.pan[p].w;
+      const std::vector<int>& cs = pk.col_slice[p];
+      const std::vector<int>& ci = pk.col_idx[p];
+      auto cqrow = [&](int k) {
+        const size_t r = h.size();
+        h.resize(r + pw, T(0));
+        for (int j = 0; j < pw; ++j)
+          if (cs[j] >= 0) h[r + j] = (T)co[cs[j]].at(ci[j], k);
+        return r;
+      };
+      const int K = pk.pan[p].K;
+      // roles of B1 / B2: 0 zero, 2 / 3 buffer P0 / P1
+      int b1 = 0, b2 = 0;
+      if (K >= 1) {
+        steps[p].push_back({0, 0, 2, cqrow(K), 0.0, 0.0, 0.0});  // B1 = c_K Q
+        b1 = 2;
+        for (int k = K; k >= 2; --k) {
+          const int out = b2 ? b2 : (b1 == 2 ? 3 : 2);
+          steps[p].push_back({b1, b2, out, cqrow(k - 1), al, be, b2 ? 1.0 : 0.0});
+          b2 = b1;
+          b1 = out;
+        }
+      }
+      steps[p].push_back({b1, b2, 4, cqrow(0), b1 ? invd : 0.0, b1 ? -c * invd : 0.0,
+                          b2 ? 1.0 : 0.0});
+    }
+    tab.alloc(std::max<size_t>(h.size(), 1));
+    SMALGO_HIP(hipMemcpyAsync(tab.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice,
+                              cx.stream));
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  }
+
+  // slice i's n x widths[i] block (row-major, ld) into / out of the panels
+  void load(int i, const T* src, int lds) {
+    for (const auto& sg : pk.segs)
+      if (sg.slice == i) {
+        const SlicePack::Panel& P = pk.pan[sg.panel];
+        copy_block<T, T>(cx.stream, n, sg.w, src + sg.c0, lds, Qp.p + P.base + sg.off, P.w);
+      }
+  }
+  void store(int i, T* dst, int ldd) {
+    for (const auto& sg : pk.segs)
+      if (sg.slice == i) {
+        const SlicePack::Panel& P = pk.pan[sg.panel];
+        copy_block<T, T>(cx.stream, n, sg.w, Yp.p + P.base + sg.off, P.w, dst + sg.c0, ldd);
+      }
+  }
+
+  void launch(int p, const Step& st) {
+    const SlicePack::Panel& P = pk.pan[p];
+    const int pw = P.w;
+    T* Qn = Qp.p + P.base;
+    T* Yn = Yp.p + P.base;
+    auto ptr = [&](int role) -> T* {
+      switch (role) {
+        case 1: return Qn;
+        case 2: return P0.p;
+        case 3: return P1.p;
+        case 4: return Yn;
+        default: return nullptr;
+      }
+    };
+    launch_clenshaw_step_cols<T>(cx.stream, *op.A, pw, Qn, pw, tab.p + st.cq, ptr(st.x1), pw,
+                                 (T)st.a1, (T)st.b1, ptr(st.x2), pw, (T)st.s2, ptr(st.out), pw,
+                                 P.tile);
+  }
+
+  // Y-panels = p_i(Â) Q-panels for every slice.  With graph = true the
+  // launches are captured once per plan into HIP graphs of at most
+  // graph_nodes_max kernels (kernel nodes only) and replayed afterwards.
+  void run(bool graph) {
+    if (!graph) {
+      for (size_t p = 0; p < pk.pan.size(); ++p)
+        for (const Step& st : steps[p]) launch((int)p, st);
+      return;
+    }
+    if (execs.empty()) capture();
+    for (auto e : execs) SMALGO_HIP(hipGraphLaunch(e, cx.stream));
+  }
+
+  void prepare() {
+    if (execs.empty()) capture();
+  }
+
+  void capture() {
+    std::vector<std::pair<int, size_t>> all;
+    for (size_t p = 0; p < pk.pan.size(); ++p)
+      for (size_t k = 0; k < steps[p].size(); ++k) all.push_back({(int)p, k});
+    for (size_t b = 0; b < all.size(); b += graph_nodes_max) {
+      const size_t e = std::min(all.size(), b + (size_t)graph_nodes_max);
+      hipGraph_t g = nullptr;
+      SMALGO_HIP(hipStreamBeginCapture(cx.stream, hipStreamCaptureModeThreadLocal));
+      try {
+        for (size_t k = b; k < e; ++k) launch(all[k].first, steps[all[k].first][all[k].second]);
+      } catch (...) {
+        (void)hipStreamEndCapture(cx.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      SMALGO_HIP(hipStreamEndCapture(cx.stream, &g));
+      graphs.push_back(g);
+      hipGraphExec_t ex = nullptr;
+      SMALGO_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      execs.push_back(ex);
+    }
+  }
+
+  size_t launches() const {
+    size_t t = 0;
+    for (const auto& v : steps) t += v.size();
+    return t;
+  }
+  // bytes one run moves by the model of a fused step (padding included)
+  double bytes() const {
+    double b = 0;
+    for (size_t p = 0; p < pk.pan.size(); ++p)
+      b += (double)steps[p].size() * ((double)op.A->nnz * (sizeof(T) + 4) + 4.0 * (n + 1) +
+                                      4.0 * n * pk.pan[p].w * sizeof(T));
+    return b;
+  }
+  int max_width() const {
+    int w = 0;
+    for (const auto& P : pk.pan) w = std::max(w, P.w);
+    return w;
+  }
+};
+
+// ---------------------------------------------------------------------------
 // POLFED building blocks
 // ---------------------------------------------------------------------------
 // Block basis of one interval: V = [Q_1 … Q_m] row-major n x ldV (block j at
@@ -892,6 +1273,83 @@ static void lanczos_step(Ctx& cx, Dense& dn, StepWork& sw, const smalgo_polfed_p
   B.Bblk.push_back(std::move(Bj));
   B.r_prev = r;
   B.r_cur = rn;
+}
+
+// p(x) - p(a) = (x - a) q(x): Chebyshev coefficients of q (degree K-1) by
+// the backward recurrence of Clenshaw's algorithm at x = a (x T_0 = T_1,
+// x T_k = (T_{k+1} + T_{k-1})/2 give q_{m-1} = 2 c_m + 2 a q_m - q_{m+1},
+// q_0 = c_1 + a q_1 - q_2 / 2); returns p(a) = c_0 + a q_0 - q_1 / 2.
+static double cheb_divide(const std::vector<double>& c, double a, std::vector<double>& q) {
+  const int K = (int)c.size() - 1;
+  q.assign(K + 2, 0.0);
+  for (int m = K; m >= 2; --m) q[m - 1] = 2.0 * c[m] + 2.0 * a * q[m] - q[m + 1];
+  if (K >= 1) q[0] = c[1] + a * q[1] - 0.5 * q[2];
+  const double pa = c[0] + a * q[0] - 0.5 * q[1];
+  q.resize(std::max(K, 0));
+  return pa;
+}
+
+struct ResidualFilterWork {
+  DBuf<double> Rd, cqd, pd;
+  DBuf<float> Rf, Wf;
+};
+
+// Y = p(Â) Z for unit columns z_i with Rayleigh quotients θ_i, written as
+//   p(Â) z_i = p(θ̂_i) z_i + q_i(Â) r̂_i,   r̂_i = (A z_i - θ_i z_i) / d,
+// q_i = (p - p(θ̂_i)) / (x - θ̂_i), θ̂ = (θ - c) / d.  The residual r̂_i is
+// formed in FP64 (one SpMM); the degree-(K-1) part runs in FP32 with a
+// per-column coefficient row.  Its rounding is relative to |r̂_i|, which is
+// already small for Ritz vectors, so Y keeps FP64 accuracy at FP32 bytes.
+static void residual_filter(Ctx& cx, const Op& op, int k, const double* Z, int ldz,
+                            const std::vector<double>& th, const std::vector<double>& co,
+                            double smin, double smax, double* Y, int ldy, ResidualFilterWork& w) {
+  const int n = op.n;
+  if (k == 0) return;
+  const double c = 0.5 * (smin + smax), d = 0.5 * (smax - smin);
+  const int K = (int)co.size() - 1;
+  std::vector<std::vector<double>> q(k);
+  std::vector<double> pa(k), mth(k);
+  for (int i = 0; i < k; ++i) {
+    const double a = std::min(1.0, std::max(-1.0, (th[i] - c) / d));
+    pa[i] = cheb_divide(co, a, q[i]);
+    mth[i] = -th[i] / d;
+  }
+  const int ld = round_up(k, 4);
+  w.Rd.ensure((size_t)n * ld);
+  w.Rf.ensure((size_t)n * ld);
+  w.Wf.ensure((size_t)n * ld);
+  w.cqd.ensure(ld);
+  w.pd.ensure(ld);
+  SMALGO_HIP(hipMemcpyAsync(w.cqd.p, mth.data(), sizeof(double) * k, hipMemcpyHostToDevice,
+                            cx.stream));
+  SMALGO_HIP(hipMemcpyAsync(w.pd.p, pa.data(), sizeof(double) * k, hipMemcpyHostToDevice,
+                            cx.stream));
+  FilterTile t = choose_tile(sizeof(double), k, {Z, w.Rd.p, Y}, {ldz, ld, ldy});
+  t.rt = 1;
+  if (K >= 1) {
+    // r̂ = (A Z - Z diag θ) / d, then rounded to FP32
+    launch_clenshaw_step_cols<double>(cx.stream, *op.A, k, Z, ldz, w.cqd.p, Z, ldz, 1.0 / d, 0.0,
+                                      nullptr, 0, 0.0, w.Rd.p, ld, t);
+    copy_block<double, float>(cx.stream, n, k, w.Rd.p, ld, w.Rf.p, ld);
+    // W = q_i(Â) r̂_i: one slice of width k, one coefficient vector per column
+    SliceFilter<float> sf(cx, op);
+    std::vector<SliceCoeffs> cs{SliceCoeffs{nullptr, &q}};
+    sf.plan({k}, cs, smin, smax, panel_width<float>());
+    sf.load(0, w.Rf.p, ld);
+    sf.run(false);
+    sf.store(0, w.Wf.p, ld);
+    copy_block<float, double>(cx.stream, n, k, w.Wf.p, ld, Y, ldy);
+  } else {
+    fill_block<double>(cx.stream, n, k, Y, ldy, 0.0);
+  }
+  // Y = p(θ̂) ∘ Z + W   (in place on W: out = cq∘Q - s2 X2 with s2 = -1)
+  launch_clenshaw_step_cols<double>(cx.stream, *op.A, k, Z, ldz, w.pd.p, nullptr, 0, 0.0, 0.0, Y,
+                                    ldy, -1.0, Y, ldy, t);
+}
+
+static bool refine_fp64_env() {
+  const char* e = std::getenv("SMALGO_REFINE_FP64");
+  return e && e[0] == '1';
 }
 
 struct ExtractStats {
@@ -1023,10 +1481,19 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
         DBuf<double> Cl, glw;
         DBuf<double> pb2;
         if (op.is_callback()) pb2.alloc((size_t)n * ldz);
+        ResidualFilterWork rfw;
+        const bool fp64_filter = refine_fp64_env() || op.is_callback();
         for (int it = 0; it < p.refine_maxiter && kz > 0; ++it) {
-          // Y = p_r(Â) Z
-          apply_filter<double>(cx, op, kz, Z.p, ldz, cr, smin, smax, Ytmp.p, ldz, Zb0.p, Zb1.p, ldz,
-                               pb2.p);
+          // Y = p_r(Â) Z: in residual form with an FP32 filter (SMALGO_REFINE_FP64=1:
+          // the FP64 filter on Z itself)
+          if (fp64_filter) {
+            apply_filter<double>(cx, op, kz, Z.p, ldz, cr, smin, smax, Ytmp.p, ldz, Zb0.p, Zb1.p,
+                                 ldz, pb2.p);
+          } else {
+            std::vector<double> th(kz);
+            for (int i = 0; i < kz; ++i) th[i] = it == 0 ? lam[zc[i]] : lam_act[i];
+            residual_filter(cx, op, kz, Z.p, ldz, th, cr, smin, smax, Ytmp.p, ldz, rfw);
+          }
           refine_steps += (long long)Kr * kz;
           // project out the locked vectors
           if (nlock > 0) {
@@ -1455,366 +1922,6 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
   return out;
 }
 
-// ---------------------------------------------------------------------------
-// Batched spectral slices (no reference counterpart: polfed, polfed.jl:1128,
-// treats one interval per call).  The current blocks of many intervals are
-// packed column-wise into dense n x pw panels and filtered together: every
-// column carries its own Chebyshev coefficients, so one launch serves
-// several slices, and narrow slices share the gathered rows of a wide panel.
-// ---------------------------------------------------------------------------
-struct SlicePack {
-  struct Seg {
-    int slice, c0, w, panel, off;  // slice columns [c0, c0+w) -> panel column off
-  };
-  struct Panel {
-    int w = 0, K = 0;  // width (multiple of 4) and degree (max over its slices)
-    size_t base = 0;   // element offset of the panel's n x w slab (set by the filter)
-    FilterTile tile{};
-  };
-  std::vector<Seg> segs;
-  std::vector<Panel> pan;
-  std::vector<std::vector<int>> col_slice;  // slice of every panel column (-1: padding)
-  int wmax = 0;
-  long long steps = 0;  // Σ_p K_p
-  double pred_ms = 0;   // modelled time of one run
-};
-
-// Packing.  Slices are taken in order of decreasing degree.  With a fixed
-// width every panel is wfix wide and a slice that does not fit the room left
-// continues in the next panel.  Otherwise panels have their own widths (up
-// to wmax, multiples of 4): a slice joins the open panel when the measured
-// cost of widening it, K_p (t(w+s) - t(w)), is below the cost of a panel of
-// its own, K_i t(s), where t(w) is the tuned time of one step at width w.
-// Joining shares the matrix read of each step and widens the gathered rows;
-// its price is the steps the joining columns wait through leading zero
-// coefficients (K_p - K_i).
-template <typename TimeFn>
-static SlicePack pack_slices(const std::vector<int>& widths, const std::vector<int>& degrees,
-                             int wmax, int wfix, TimeFn&& t) {
-  SlicePack pk;
-  pk.wmax = wfix > 0 ? wfix : wmax;
-  const int ns = (int)widths.size();
-  std::vector<int> ord(ns);
-  for (int i = 0; i < ns; ++i) ord[i] = i;
-  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return degrees[a] > degrees[b]; });
-  int used = 0;  // columns used in the open panel (-1: none open)
-  bool open = false;
-  auto start = [&](int K) {
-    pk.pan.push_back({});
-    pk.pan.back().K = K;
-    pk.col_slice.emplace_back();
-    used = 0;
-    open = true;
-  };
-  for (int i : ord) {
-    int c0 = 0;
-    while (c0 < widths[i]) {
-      const int rem = widths[i] - c0;
-      bool join = false;
-      if (open && used < pk.wmax) {
-        if (wfix > 0) {
-          join = true;
-        } else {
-          const int take = std::min(rem, pk.wmax - used);
-          const int Kp = pk.pan.back().K;
-          const double widen = Kp * (t(round_up(used + take, 4)) - t(round_up(used, 4)));
-          const double alone = degrees[i] * t(round_up(take, 4));
-          join = widen <= alone;
-        }
-      }
-      if (!join) start(degrees[i]);
-      const int w = std::min(rem, pk.wmax - used);
-      const int panel = (int)pk.pan.size() - 1;
-      pk.segs.push_back({i, c0, w, panel, used});
-      for (int c = 0; c < w; ++c) pk.col_slice[panel].push_back(i);
-      used += w;
-      c0 += w;
-      if (used == pk.wmax) open = false;
-    }
-  }
-  for (size_t p = 0; p < pk.pan.size(); ++p) {
-    SlicePack::Panel& P = pk.pan[p];
-    P.w = wfix > 0 ? wfix : round_up((int)pk.col_slice[p].size(), 4);
-    pk.col_slice[p].resize(P.w, -1);
-    pk.steps += P.K;
-    pk.pred_ms += P.K * t(P.w);
-  }
-  return pk;
-}
-
-template <typename T>
-struct SliceFilter {
-  // One fused step of a panel: out = cq∘Q + a1 (A X1) + b1 X1 - s2 X2 with a
-  // per-column cq row; operands by role (0 none, 1 Q, 2 P0, 3 P1, 4 Y).
-  struct Step {
-    int x1, x2, out;
-    size_t cq;  // table offset of the cq row
-    double a1, b1, s2;
-  };
-  Ctx& cx;
-  const Op& op;
-  int n = 0;
-  SlicePack pk;
-  std::vector<int> widths;
-  DBuf<T> Qp, Yp, P0, P1, tab;  // panels, ping-pong buffers, coefficient table
-  std::vector<std::vector<Step>> steps;
-  std::vector<hipGraphExec_t> execs;
-  std::vector<hipGraph_t> graphs;
-  int graph_nodes_max = 2048;  // launches per captured graph
-
-  SliceFilter(Ctx& c, const Op& o) : cx(c), op(o), n(o.n) {}
-  SliceFilter(const SliceFilter&) = delete;
-  SliceFilter& operator=(const SliceFilter&) = delete;
-  ~SliceFilter() { drop_graphs(); }
-
-  void drop_graphs() {
-    for (auto e : execs) (void)hipGraphExecDestroy(e);
-    for (auto g : graphs) (void)hipGraphDestroy(g);
-    execs.clear();
-    graphs.clear();
-  }
-
-  // Build the packing, the per-column coefficient table and the step list.
-  // wfix > 0 gives every panel that width; otherwise panels are sized by the
-  // measured step time of each width (up to 512-byte rows).
-  void plan(const std::vector<int>& w, const std::vector<const std::vector<double>*>& co,
-            double smin, double smax, int wfix) {
-    drop_graphs();
-    widths = w;
-    std::vector<int> deg(w.size());
-    for (size_t i = 0; i < w.size(); ++i) deg[i] = (int)co[i]->size() - 1;
-    const int wmax = wfix > 0 ? wfix : panel_width<T>();
-    P0.ensure((size_t)n * wmax);
-    P1.ensure((size_t)n * wmax);
-    auto t = [&](int width) {
-      if (width <= 0) return 0.0;
-      float ms = 0;
-      tune(width, &ms);
-      return (double)ms;
-    };
-    pk = pack_slices(w, deg, panel_width<T>(), wfix, t);
-    for (auto& P : pk.pan) {
-      float ms = 0;
-      P.tile = tune(P.w, &ms);
-    }
-    build_table(co, smin, smax);
-    size_t tot = 0;
-    for (auto& P : pk.pan) {  // slabs one after another
-      P.base = tot;
-      tot += (size_t)n * P.w;
-    }
-    Qp.ensure(tot);
-    Yp.ensure(tot);
-    for (const auto& P : pk.pan) fill_block<T>(cx.stream, n, P.w, Qp.p + P.base, P.w, T(0));
-    fill_block<T>(cx.stream, n, wmax, P0.p, wmax, T(0));
-    fill_block<T>(cx.stream, n, wmax, P1.p, wmax, T(0));
-    SMALGO_HIP(hipStreamSynchronize(cx.stream));
-  }
-
-  // Autotuner for the per-column kernel at width w: time a few steps of
-  // every (vec, lanes per row, rows per team) tile on the real matrix.
-  FilterTile tune(int w, float* ms_out) {
-    const long long key = ((long long)sizeof(T) << 56) ^ ((long long)w << 32) ^
-                          ((long long)op.mat_id << 4) ^ (1LL << 3);
-    auto it = cx.tuned.find(key);
-    if (it != cx.tuned.end() && cx.tuned_ms.count(key)) {
-      *ms_out = cx.tuned_ms[key];
-      return it->second;
-    }
-    DBuf<T> co((size_t)w);
-    fill_block<T>(cx.stream, 1, w, co.p, w, T(0.25));
-    fill_block<T>(cx.stream, n, w, P0.p, w, T(0));
-    fill_block<T>(cx.stream, n, w, P1.p, w, T(0));
-    std::vector<FilterTile> cand;
-    FilterTile forced{};
-    if (env_tile("SMALGO_SLICE_TILE", &forced) && w % forced.vec == 0) {
-      cand.push_back(forced);
-    } else if (!cx.autotune) {
-      const FilterTile d = choose_tile(sizeof(T), w, {P0.p, P1.p}, {w});
-      cand.push_back(FilterTile{d.vec, d.lpr, 1});
-    } else {
-      for (int v : {1, 2, 4})
-        for (int l : {4, 8, 16, 32, 64})
-          if (w % v == 0 && l * v <= std::max(w, 4) * 2 && l * v * (int)sizeof(T) <= 2048)
-            for (int rt : {1, 2, 4}) cand.push_back(FilterTile{v, l, rt});
-    }
-    hipEvent_t e0, e1;
-    SMALGO_HIP(hipEventCreate(&e0));
-    SMALGO_HIP(hipEventCreate(&e1));
-    FilterTile best = cand.front();
-    float best_ms = 1e30f;
-    for (const FilterTile& tl : cand) {
-      // the general step out = cq∘Q + a1 A X1 + b1 X1 - X2, in place on X2
-      launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, P1.p, w, co.p, P0.p, w, T(0.1), T(0.1),
-                                   P1.p, w, T(1), P1.p, w, tl);
-      SMALGO_HIP(hipEventRecord(e0, cx.stream));
-      for (int r = 0; r < 4; ++r)
-        launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, P1.p, w, co.p, P0.p, w, T(0.1), T(0.1),
-                                     P1.p, w, T(1), P1.p, w, tl);
-      SMALGO_HIP(hipEventRecord(e1, cx.stream));
-      SMALGO_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms / 4 < best_ms) {
-        best_ms = ms / 4;
-        best = tl;
-      }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    cx.tuned[key] = best;
-    cx.tuned_ms[key] = best_ms;
-    *ms_out = best_ms;
-    return best;
-  }
-
-  // The Clenshaw recurrence of apply_poly_block! (polfed.jl:464-501) for
-  // every panel, unfolded: B_{K+1} = c_K Q is materialised by a Q-only step
-  // (no gather), then TMP = c_{k-1} Q + α A B1 + β B1 - B2 for k = K..2 with
-  // TMP overwriting B2 in place once B2 exists, then the final
-  // Y = c_0 Q + invd A B1 - c invd B1 - B2.  Only the Q coefficient differs
-  // between columns (all slices share smin/smax), so one row of cq per step.
-  // A column whose slice has degree K_i below the panel's K_p gets zero
-  // coefficients for k > K_i: its state stays exactly zero until step K_i,
-  // so it runs exactly the degree-K_i recurrence of its own slice.  Padding
-  // columns have Q = 0 and cq = 0 and stay zero.
-  void build_table(const std::vector<const std::vector<double>*>& co, double smin, double smax) {
-    const double c = 0.5 * (smin + smax);
-    const double invd = 2.0 / (smax - smin);
-    const double al = 2.0 * invd, be = -2.0 * c * invd;
-    std::vector<T> h;
-    steps.assign(pk.pan.size(), {});
-    for (size_t p = 0; p < pk.pan.size(); ++p) {
-      const int pw = pk.pan[p].w;
-      const std::vector<int>& cs = pk.col_slice[p];
-      auto cqrow = [&](int k) {
-        const size_t r = h.size();
-        h.resize(r + pw, T(0));
-        for (int j = 0; j < pw; ++j) {
-          const int sl = cs[j];
-          if (sl < 0) continue;
-          const std::vector<double>& v = *co[sl];
-          if (k < (int)v.size()) h[r + j] = (T)v[k];
-        }
-        return r;
-      };
-      const int K = pk.pan[p].K;
-      // roles of B1 / B2: 0 zero, 2 / 3 buffer P0 / P1
-      int b1 = 0, b2 = 0;
-      if (K >= 1) {
-        steps[p].push_back({0, 0, 2, cqrow(K), 0.0, 0.0, 0.0});  // B1 = c_K Q
-        b1 = 2;
-        for (int k = K; k >= 2; --k) {
-          const int out = b2 ? b2 : (b1 == 2 ? 3 : 2);
-          steps[p].push_back({b1, b2, out, cqrow(k - 1), al, be, b2 ? 1.0 : 0.0});
-          b2 = b1;
-          b1 = out;
-        }
-      }
-      steps[p].push_back({b1, b2, 4, cqrow(0), b1 ? invd : 0.0, b1 ? -c * invd : 0.0,
-                          b2 ? 1.0 : 0.0});
-    }
-    tab.alloc(std::max<size_t>(h.size(), 1));
-    SMALGO_HIP(hipMemcpyAsync(tab.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice,
-                              cx.stream));
-    SMALGO_HIP(hipStreamSynchronize(cx.stream));
-  }
-
-  // slice i's n x widths[i] block (row-major, ld) into / out of the panels
-  void load(int i, const T* src, int lds) {
-    for (const auto& sg : pk.segs)
-      if (sg.slice == i) {
-        const SlicePack::Panel& P = pk.pan[sg.panel];
-        copy_block<T, T>(cx.stream, n, sg.w, src + sg.c0, lds, Qp.p + P.base + sg.off, P.w);
-      }
-  }
-  void store(int i, T* dst, int ldd) {
-    for (const auto& sg : pk.segs)
-      if (sg.slice == i) {
-        const SlicePack::Panel& P = pk.pan[sg.panel];
-        copy_block<T, T>(cx.stream, n, sg.w, Yp.p + P.base + sg.off, P.w, dst + sg.c0, ldd);
-      }
-  }
-
-  void launch(int p, const Step& st) {
-    const SlicePack::Panel& P = pk.pan[p];
-    const int pw = P.w;
-    T* Qn = Qp.p + P.base;
-    T* Yn = Yp.p + P.base;
-    auto ptr = [&](int role) -> T* {
-      switch (role) {
-        case 1: return Qn;
-        case 2: return P0.p;
-        case 3: return P1.p;
-        case 4: return Yn;
-        default: return nullptr;
-      }
-    };
-    launch_clenshaw_step_cols<T>(cx.stream, *op.A, pw, Qn, pw, tab.p + st.cq, ptr(st.x1), pw,
-                                 (T)st.a1, (T)st.b1, ptr(st.x2), pw, (T)st.s2, ptr(st.out), pw,
-                                 P.tile);
-  }
-
-  // Y-panels = p_i(Â) Q-panels for every slice.  With graph = true the
-  // launches are captured once per plan into HIP graphs of at most
-  // graph_nodes_max kernels (kernel nodes only) and replayed afterwards.
-  void run(bool graph) {
-    if (!graph) {
-      for (size_t p = 0; p < pk.pan.size(); ++p)
-        for (const Step& st : steps[p]) launch((int)p, st);
-      return;
-    }
-    if (execs.empty()) capture();
-    for (auto e : execs) SMALGO_HIP(hipGraphLaunch(e, cx.stream));
-  }
-
-  void prepare() {
-    if (execs.empty()) capture();
-  }
-
-  void capture() {
-    std::vector<std::pair<int, size_t>> all;
-    for (size_t p = 0; p < pk.pan.size(); ++p)
-      for (size_t k = 0; k < steps[p].size(); ++k) all.push_back({(int)p, k});
-    for (size_t b = 0; b < all.size(); b += graph_nodes_max) {
-      const size_t e = std::min(all.size(), b + (size_t)graph_nodes_max);
-      hipGraph_t g = nullptr;
-      SMALGO_HIP(hipStreamBeginCapture(cx.stream, hipStreamCaptureModeThreadLocal));
-      try {
-        for (size_t k = b; k < e; ++k) launch(all[k].first, steps[all[k].first][all[k].second]);
-      } catch (...) {
-        (void)hipStreamEndCapture(cx.stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
-      }
-      SMALGO_HIP(hipStreamEndCapture(cx.stream, &g));
-      graphs.push_back(g);
-      hipGraphExec_t ex = nullptr;
-      SMALGO_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-      execs.push_back(ex);
-    }
-  }
-
-  size_t launches() const {
-    size_t t = 0;
-    for (const auto& v : steps) t += v.size();
-    return t;
-  }
-  // bytes one run moves by the model of a fused step (padding included)
-  double bytes() const {
-    double b = 0;
-    for (size_t p = 0; p < pk.pan.size(); ++p)
-      b += (double)steps[p].size() * ((double)op.A->nnz * (sizeof(T) + 4) + 4.0 * (n + 1) +
-                                      4.0 * n * pk.pan[p].w * sizeof(T));
-    return b;
-  }
-  int max_width() const {
-    int w = 0;
-    for (const auto& P : pk.pan) w = std::max(w, P.w);
-    return w;
-  }
-};
-
 static int slice_pw_env() {
   const char* e = std::getenv("SMALGO_SLICE_PW");
   return e ? std::atoi(e) : 0;
@@ -1828,8 +1935,8 @@ static void apply_poly_slices_t(Ctx& cx, const Op& op, const std::vector<int>& w
   const int n = op.n, ns = (int)widths.size();
   int wtot = 0;
   for (int w : widths) wtot += w;
-  std::vector<const std::vector<double>*> co;
-  for (const auto& v : coeffs) co.push_back(&v);
+  std::vector<SliceCoeffs> co(ns);
+  for (int i = 0; i < ns; ++i) co[i].shared = &coeffs[i];
   SliceFilter<T> sf(cx, op);
   sf.plan(widths, co, smin, smax, pw_force);
   const int ld = round_up(std::max(wtot, 1), 4);
@@ -1949,10 +2056,10 @@ static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
     if (act.empty()) break;
     if (act != planned) {
       std::vector<int> w;
-      std::vector<const std::vector<double>*> co;
+      std::vector<SliceCoeffs> co;
       for (int i : act) {
         w.push_back(sl[i].s);
-        co.push_back(&coeffs[i]);
+        co.push_back(SliceCoeffs{&coeffs[i], nullptr});
       }
       sf.plan(w, co, smin, smax, pw_force);
       planned = act;

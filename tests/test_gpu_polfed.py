@@ -100,3 +100,32 @@ def test_polfed_unknown_window(ctx):
     import sparse_matrix_algorithms_amd as sm
     with pytest.raises(ValueError, match="Unknown window type"):
         sm.polfed(lap1d(20), 2, 10, s=2, window_type="box", ctx=ctx)
+
+
+def test_mixed_refine_residual_form_matches_fp64_filter(ctx, monkeypatch):
+    """The mixed path's FP64 refinement filters residuals in FP32 (p(Â)z =
+    p(θ̂)z + q(Â)r̂); against the plain FP64 refinement filter (switch
+    SMALGO_REFINE_FP64=1) it returns the same pairs at the same accuracy."""
+    import sparse_matrix_algorithms_amd as sm
+    A = anderson3d(12, 16.5, 8)
+    n = A.shape[0]
+    ev = np.linalg.eigh(A.toarray())[0]
+    nev = 60
+    emin, emax = _window(ev, nev)
+    exact = ev[(ev >= emin) & (ev <= emax)]
+    smin, smax = P.gershgorin_bounds(A)
+    anorm = max(abs(smin), abs(smax))
+    kw = dict(s=16, smin=smin, smax=smax, emin=emin, emax=emax, mmax=math.ceil(3 * nev / 16),
+              precision="fp32", return_stats=True, ctx=ctx, Q1=gaussian_block(2, n, 16))
+    K = 700
+    lam_r, X_r, res_r, st_r = sm.polfed(A, nev, K, **kw)
+    monkeypatch.setenv("SMALGO_REFINE_FP64", "1")
+    lam_d, _, res_d, st_d = sm.polfed(A, nev, K, **kw)
+    monkeypatch.delenv("SMALGO_REFINE_FP64")
+    for lam, res in ((lam_r, res_r), (lam_d, res_d)):
+        assert len(lam) == nev
+        assert np.max(np.abs(np.sort(lam) - exact)) <= 1e-10 * anorm
+        assert res.max() <= 1e-10 * anorm
+    assert st_r["refine_filter_steps"] > 0 and st_d["refine_filter_steps"] > 0
+    R = A @ X_r - X_r * lam_r[None, :]
+    assert np.max(np.linalg.norm(R, axis=0)) <= 2e-10 * anorm
