@@ -198,7 +198,8 @@ typedef struct smalgo_polfed_params {
   int show_progress;     /* default 0: print the progress line (polfed.jl:991-1000) */
   /* mixed precision (north star; no reference counterpart) */
   int precision;         /* SMALGO_PREC_* (default FP64) */
-  int refine_guard;      /* guard vectors in FP64 refinement (default -1: nev/2) */
+  int refine_guard;      /* guard vectors in FP64 refinement (default -1: nev, at most the
+                            Ritz vectors left after the candidates) */
   int refine_degree;     /* degree of the FP64 refinement filter (default -1: degree) */
   int refine_maxiter;    /* FP64 refinement iterations with locking (default 3) */
   double refine_tol;     /* relative residual target for locking (default 1e-10 * 0.5) */

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void clenshaw_kernel(
     if (gather) {
       // all index, value and gather loads unconditional (clamped index, zero
       // weight past the row end) so the RT*U load chains of a lane overlap
-      constexpr int U = RT >= 4 ? 1 : 2;
+      constexpr int U = RT >= 4 ? 1 : (RT == 2 ? 2 : 4);
       for (int j = 0; j < maxlen; j += U) {
         int col[U][RT];
         T w[U][RT];

@@ -1457,7 +1457,7 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
         // next Ritz vectors by θ), Rayleigh–Ritz in FP64, locking of pairs whose
         // residual meets refine_tol·‖A‖ (‖A‖ = max(|smin|,|smax|)).
         const size_t hr = tm.begin("refine");
-        const int guard = p.refine_guard >= 0 ? p.refine_guard : nev / 2;
+        const int guard = p.refine_guard >= 0 ? p.refine_guard : nev;
         std::vector<int> zc(idxc);
         std::vector<char> used(m, 0);
         for (int v : idxc) used[v] = 1;
