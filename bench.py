@@ -132,10 +132,14 @@ def main():
                                    args.mmax_factor)
     from sparse_matrix_algorithms_amd.learned import MLP, choose_parameters
     model = None if args.no_learned else MLP.load(ctx=ctx)
-    KL, mmaxL, src = choose_parameters(model, mu[:64], n, args.nev, emin, emax, smin, smax, args.s)
+    head_prec = "fp64" if args.headline_fp64 else "fp32"
+    KL, mmaxL, src = choose_parameters(model, mu[:64], n, args.nev, emin, emax, smin, smax, args.s,
+                                       precision=head_prec)
+    KL64, mmaxL64, _ = choose_parameters(model, mu[:64], n, args.nev, emin, emax, smin, smax,
+                                         args.s)
     log(f"[bench] KPM({args.kpm_moments} moments, {args.kpm_probes} probes) interval "
         f"[{emin:.6f},{emax:.6f}] in {t_k:.2f}s; plain rule: degree={K} mmax={mmax}; "
-        f"{src}: degree={KL} mmax={mmaxL}")
+        f"{src} ({head_prec}): degree={KL} mmax={mmaxL}")
 
     common = dict(s=args.s, smin=smin, smax=smax, sigma=0.0, emin=emin, emax=emax,
                   tol=1e-10 * anorm, ctx=ctx, return_stats=True, seed=1)
@@ -150,7 +154,6 @@ def main():
                                     **extra)
         return lam, X, res, st, time.perf_counter() - t
 
-    head_prec = "fp64" if args.headline_fp64 else "fp32"
     for w in range(args.warmup):
         _, _, _, st, dt = solve(head_prec, KL, mmaxL)
         log(f"[bench] warmup {w}: {dt:.3f}s {st['line']}")
@@ -215,8 +218,8 @@ def main():
         log(f"[bench] fp64 plain: {dt64:.3f}s {st64['line']}")
         st64_steps = st64["filter_steps"]
         if not args.skip_variants:
-            l2, _, r2, s2, d2 = solve("fp64", KL, mmaxL)
-            detail["fp64_" + src], set2 = summary(l2, r2, s2, d2, KL, mmaxL)
+            l2, _, r2, s2, d2 = solve("fp64", KL64, mmaxL64)
+            detail["fp64_" + src], set2 = summary(l2, r2, s2, d2, KL64, mmaxL64)
             detail["fp64_" + src]["same_set_as_fp64_plain"] = same(set2, set64)
             log(f"[bench] fp64 {src}: {d2:.3f}s {s2['line']}")
             l3, _, r3, s3, d3 = solve("fp32", K, mmax)

@@ -40,6 +40,7 @@ def margin_bucket(blocks: float) -> int:
 KAPPA_RANGE = (3.0, 10.0)
 FACTOR_RANGE = (1.6, 7.0)
 WEIGHTS = Path(__file__).resolve().parent / "data" / "mlp_weights.json"
+MIXED_DEGREE_SCALE = 0.8   # learned degree factor for the mixed path
 
 
 def _fp(a):
@@ -146,10 +147,14 @@ def decode_targets(y) -> tuple[float, float]:
 
 def choose_parameters(model: Optional[MLP], mu: np.ndarray, n: int, nev: int, emin: float,
                       emax: float, smin: float, smax: float, s: int, margin_kappa=None,
-                      margin_factor=None):
+                      margin_factor=None, precision: str = "fp64"):
     """(degree, mmax, source) from the network plus its calibrated safety
     margins, or from the plain rule when the predicted basis undercounts the
-    stochastic estimate (or no model)."""
+    stochastic estimate (or no model).  For the mixed path (precision
+    "fp32") the learned degree is scaled by MIXED_DEGREE_SCALE: its FP64
+    refinement is one more filtered subspace step on the candidates, so the
+    FP32 basis converges with a lower degree (held-out check in
+    data/heldout_eval_mixed.json)."""
     K0, m0 = plain_parameters(n, nev, emin, emax, smin, smax, s)
     if model is None:
         return K0, m0, "plain"
@@ -167,4 +172,6 @@ def choose_parameters(model: Optional[MLP], mu: np.ndarray, n: int, nev: int, em
     est = kpm_count(mu, n, emin, emax, smin, smax)
     if mmax * s < 2.0 * est:
         return K0, m0, "fallback"
+    if precision == "fp32":
+        K = max(1, math.ceil(MIXED_DEGREE_SCALE * K))
     return K, mmax, "learned"

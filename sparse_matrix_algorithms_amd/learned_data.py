@@ -42,11 +42,11 @@ def instance(i: int, base: int = 2026):
                 target=int(rng.integers(40, 241)))
 
 
-def _solve(sm, dm, nev, K, mmax, emin, emax, smin, smax, anorm, ctx):
+def _solve(sm, dm, nev, K, mmax, emin, emax, smin, smax, anorm, ctx, precision="fp64"):
     t = time.perf_counter()
     lam, _, res, st = sm.polfed(dm, nev, K, s=S, smin=smin, smax=smax, emin=emin, emax=emax,
                                 sigma=0.5 * (emin + emax), mmax=mmax, tol=1e-10 * anorm, ctx=ctx,
-                                return_stats=True, want_X=False)
+                                return_stats=True, want_X=False, precision=precision)
     dt = time.perf_counter() - t
     ok = (lam >= emin) & (lam <= emax) & (res <= 1e-10 * anorm)
     return np.sort(lam[ok]), dt, st
@@ -154,9 +154,13 @@ def train(path: Path, epochs: int = 3000, lr: float = 3e-3, out: Path = None):
     return model, hist
 
 
-def evaluate(path: Path, weights: Path = None, margin_kappa=None, margin_factor=None):
+def evaluate(path: Path, weights: Path = None, margin_kappa=None, margin_factor=None,
+             precision: str = "fp64", degree_scale: float = 1.0):
     """Held-out instances (256..319): learned parameters (with the stochastic
-    count fallback) versus the plain rule, both solved on the GPU in FP64.
+    count fallback) versus the plain rule solved on the GPU in FP64.  The
+    learned run uses `precision` and the learned degree times `degree_scale`
+    (the mixed path's FP64 refinement is one more filtered subspace step, so
+    its basis can use a lower degree).
     A run succeeds when it returns at least 0.9 of the KPM count of the
     window at relative residual ≤ 1e-10 and, where the plain run succeeded,
     the same eigenvalue set as the plain run."""
@@ -173,8 +177,10 @@ def evaluate(path: Path, weights: Path = None, margin_kappa=None, margin_factor=
         mu = np.array(r["mu"])
         K, mmax, src = choose_parameters(model, mu, r["n"], r["nev"], r["emin"], r["emax"],
                                          r["smin"], r["smax"], S, margin_kappa, margin_factor)
+        if src != "fallback":  # the scale is applied here, on the FP64 rule's degree
+            K = max(1, math.ceil(degree_scale * K))
         lam, t, st = _solve(sm, dm, r["nev"], K, mmax, r["emin"], r["emax"], r["smin"],
-                            r["smax"], anorm, ctx)
+                            r["smax"], anorm, ctx, precision)
         K0, m0 = r["plain"]["degree"], r["plain"]["mmax"]
         lam0, t0, st0 = _solve(sm, dm, r["nev"], K0, m0, r["emin"], r["emax"], r["smin"],
                                r["smax"], anorm, ctx)
@@ -191,7 +197,7 @@ def evaluate(path: Path, weights: Path = None, margin_kappa=None, margin_factor=
               f"(K={K}, mmax={mmax}) plain {t0:.3f}s (K={K0}, mmax={m0})", flush=True)
         dm.free()
     both = [x for x in rows if x["plain_ok"]]
-    summ = dict(instances=len(rows),
+    summ = dict(instances=len(rows), precision=precision, degree_scale=degree_scale,
                 success_learned=sum(x["ok"] for x in rows),
                 success_plain=sum(x["plain_ok"] for x in rows),
                 fallback_rate=sum(x["source"] == "fallback" for x in rows) / max(len(rows), 1),
@@ -216,6 +222,10 @@ def main():
     ap.add_argument("--out", type=Path, default=None)
     ap.add_argument("--epochs", type=int, default=3000)
     ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64",
+                    help="eval: precision of the learned-parameter runs")
+    ap.add_argument("--degree-scale", type=float, default=1.0,
+                    help="eval: factor on the learned degree")
     ap.add_argument("--redo-unconverged", type=Path, default=None,
                     help="regenerate the instances of this log whose plain run did not converge")
     a = ap.parse_args()
@@ -229,7 +239,7 @@ def main():
     elif a.cmd == "train":
         train(a.data, a.epochs, a.lr, a.out)
     else:
-        summ, rows = evaluate(a.data)
+        summ, rows = evaluate(a.data, precision=a.precision, degree_scale=a.degree_scale)
         if a.out:
             Path(a.out).write_text(json.dumps(dict(summary=summ, rows=rows), indent=1))
 
