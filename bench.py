@@ -177,6 +177,8 @@ def main():
                     t_filter=st_["t_filter"], t_orth=st_["t_orth"], t_extract=st_["t_extract"],
                     t_refine=st_.get("t_refine", 0.0),
                     refine_filter_steps=st_.get("refine_filter_steps", 0),
+                    refine_iterations=st_.get("refine_iterations", 0),
+                    t_refine_host=st_.get("t_refine_host", 0.0),
                     locked=st_.get("locked", 0),
                     filter_gbs=round(st_["spmm_bytes"] / st_["t_filter"] / 1e9, 1),
                     orth_gemm_tflops=round(st_["gemm_flops"] / max(st_["t_orth"], 1e-9) / 1e12,

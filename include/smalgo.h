@@ -249,7 +249,8 @@ int smalgo_polfed_slices(smalgo_ctx* ctx, const smalgo_mat* A, int nslices,
  * (leading dimension ldq / ldy, slice i at the running column offset); block r
  * goes through the same plan (and, with use_graph, the same instantiated
  * graphs) as block 0.  panel_width > 0 gives every panel that width (a
- * multiple of 4); 0 sizes each panel (up to 512-byte rows) from the measured
+ * multiple of 4); 0 sizes each panel (rows of at most 512 B, less where three
+ * panel blocks would not fit the 256 MB Infinity Cache) from the measured
  * step time of each width.  stats (optional, 6 doubles): widest panel,
  * panels, launches per run, ms per run, modelled bytes per run, graphs per run. */
 int smalgo_apply_poly_slices(smalgo_ctx* ctx, const smalgo_mat* A, int nslices, const int* widths,

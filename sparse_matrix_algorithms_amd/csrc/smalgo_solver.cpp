@@ -296,9 +296,17 @@ static void clenshaw_recurrence(const std::vector<double>& co, double smin, doub
   step(co[0], invd, -c * invd, B1, B2, Y);
 }
 
-// panel width for wide blocks: 512-byte row segments per slab
+// Cache-aware panel width: the widest of 512 / 256 / 128-byte row segments
+// whose recurrence working set (Q, B1, B2 of one panel plus the matrix) fits
+// in 240 MB of the 256 MB Infinity Cache.  Measured at n = 262,144: 256-byte
+// panels (201 MB) run at 5.7 TB/s, 512-byte panels (403 MB) at 3.8 TB/s.
 template <typename T>
-static constexpr int panel_width() { return 512 / (int)sizeof(T); }
+static int cache_panel_width(const Op& op) {
+  const double mat = op.A ? (double)op.A->nnz * (sizeof(T) + 4) : 0.0;
+  for (int bytes = 512; bytes > 128; bytes /= 2)
+    if (3.0 * op.n * bytes + mat <= 240e6) return bytes / (int)sizeof(T);
+  return 128 / (int)sizeof(T);
+}
 
 // Tuning overrides (environment): SMALGO_FILTER_TILE="vec,lpr,rt" fixes the
 // tile of row-major filters, SMALGO_PANEL_TILE likewise for panel filters
@@ -319,18 +327,18 @@ void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::
                   double* pbuf) {
   const char* nop = std::getenv("SMALGO_NO_PANELS");
   const bool panels_ok = !(nop && nop[0] == '1');
-  if (!op.is_callback() && panels_ok && b > panel_width<T>()) {
-    // Wide blocks: repack into dense n x PW slabs so that every gather and
-    // stream of one launch chunk walks contiguous rows (DRAM page locality),
-    // then unpack the result.
-    const int pw = panel_width<T>();
+  if (!op.is_callback() && panels_ok && b > cache_panel_width<T>(op)) {
+    // Wide blocks: repack into dense n x PW slabs (PW from the cache-aware
+    // rule) so that every gather and stream of one launch chunk walks
+    // contiguous rows, then unpack the result.
+    const int pw = cache_panel_width<T>(op);
     const int npan = (b + pw - 1) / pw;
     const int n = op.n;
     const size_t ps = (size_t)n * pw;
     DBuf<T> Qp(ps * npan), P0(ps), P1(ps), Yp(ps * npan);
     to_panels<T>(cx.stream, n, b, pw, Q, ldq, Qp.p);
     // Panels run one after another through the whole recurrence (panel-outer,
-    // step-inner): one panel's Q, B1, B2 (3 x n x 512 B) stay resident in the
+    // step-inner): one panel's Q, B1, B2 (3 x n x PW) stay resident in the
     // 256 MB Infinity Cache across its K steps, which interleaving the panels
     // step by step would evict.  Each launch has the narrow-block shape the
     // autotuner measured.
@@ -910,14 +918,14 @@ struct SliceFilter {
 
   // Build the packing, the per-column coefficient table and the step list.
   // wfix > 0 gives every panel that width; otherwise panels are sized by the
-  // measured step time of each width (up to 512-byte rows).
+  // measured step time of each width, up to the cache-aware panel width.
   void plan(const std::vector<int>& w, const std::vector<SliceCoeffs>& co, double smin,
             double smax, int wfix) {
     drop_graphs();
     widths = w;
     std::vector<int> deg(w.size());
     for (size_t i = 0; i < w.size(); ++i) deg[i] = co[i].degree();
-    const int wmax = wfix > 0 ? wfix : panel_width<T>();
+    const int wmax = wfix > 0 ? wfix : cache_panel_width<T>(op);
     P0.ensure((size_t)n * wmax);
     P1.ensure((size_t)n * wmax);
     auto t = [&](int width) {
@@ -926,7 +934,7 @@ struct SliceFilter {
       tune(width, &ms);
       return (double)ms;
     };
-    pk = pack_slices(w, deg, panel_width<T>(), wfix, t);
+    pk = pack_slices(w, deg, wmax, wfix, t);
     for (auto& P : pk.pan) {
       float ms = 0;
       P.tile = tune(P.w, &ms);
@@ -1292,6 +1300,7 @@ static double cheb_divide(const std::vector<double>& c, double a, std::vector<do
 struct ResidualFilterWork {
   DBuf<double> Rd, cqd, pd;
   DBuf<float> Rf, Wf;
+  double host_s = 0;  // host time spent on coefficients and plans
 };
 
 // Y = p(Â) Z for unit columns z_i with Rayleigh quotients θ_i, written as
@@ -1305,6 +1314,7 @@ static void residual_filter(Ctx& cx, const Op& op, int k, const double* Z, int l
                             double smin, double smax, double* Y, int ldy, ResidualFilterWork& w) {
   const int n = op.n;
   if (k == 0) return;
+  const auto h0 = std::chrono::steady_clock::now();
   const double c = 0.5 * (smin + smax), d = 0.5 * (smax - smin);
   const int K = (int)co.size() - 1;
   std::vector<std::vector<double>> q(k);
@@ -1334,7 +1344,8 @@ static void residual_filter(Ctx& cx, const Op& op, int k, const double* Z, int l
     // W = q_i(Â) r̂_i: one slice of width k, one coefficient vector per column
     SliceFilter<float> sf(cx, op);
     std::vector<SliceCoeffs> cs{SliceCoeffs{nullptr, &q}};
-    sf.plan({k}, cs, smin, smax, panel_width<float>());
+    sf.plan({k}, cs, smin, smax, cache_panel_width<float>(op));
+    w.host_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
     sf.load(0, w.Rf.p, ld);
     sf.run(false);
     sf.store(0, w.Wf.p, ld);
@@ -1356,6 +1367,8 @@ struct ExtractStats {
   long long refine_steps = 0;
   int locked = 0;
   double gemm_flops = 0;
+  int refine_iterations = 0;
+  double refine_host_s = 0;
 };
 
 // extract_pairs (polfed.jl:892-984) + the return rule of polfed (:1148-1157)
@@ -1495,6 +1508,7 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
             residual_filter(cx, op, kz, Z.p, ldz, th, cr, smin, smax, Ytmp.p, ldz, rfw);
           }
           refine_steps += (long long)Kr * kz;
+          ++xs.refine_iterations;
           // project out the locked vectors
           if (nlock > 0) {
             Cl.ensure((size_t)nlock * ldz);
@@ -1560,6 +1574,7 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
           if (inside_unconv == 0) break;
         }
         locked_total = nlock;
+        xs.refine_host_s = rfw.host_s;
         // candidate set = locked ∪ active; the reference's post-refine selection follows
         kc = nlock + kz;
         ldc = round_up(std::max(kc, 1), 4);
@@ -1736,6 +1751,8 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
   out->stats["t_refine"] = ph["refine"];
   out->stats["filter_steps"] = (double)filter_steps;
   out->stats["refine_filter_steps"] = (double)xs.refine_steps;
+  out->stats["refine_iterations"] = xs.refine_iterations;
+  out->stats["t_refine_host"] = xs.refine_host_s;
   out->stats["locked"] = xs.locked;
   out->stats["gemm_flops"] = gemm_flops + xs.gemm_flops;
   out->stats["smin"] = smin;
