@@ -40,10 +40,12 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("anderson", "slices", "xxz"), default="anderson",
+    ap.add_argument("--workload", choices=("anderson", "slices", "xxz", "cheb"),
+                    default="anderson",
                     help="anderson: one interval of the L=48 Anderson model (the headline); "
                          "slices: 16 intervals of the L=64 Anderson model in one batched run; "
-                         "xxz: ~1000 pairs at the centre of the L=20 random-field XXZ chain")
+                         "xxz: ~1000 pairs at the centre of the L=20 random-field XXZ chain; "
+                         "cheb: Chebyshev subspace iteration on L=20 Anderson, [-0.2, 0.2]")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default=None,
                     help="slices/xxz: arithmetic of the basis build (fp32 = FP32 filter + FP64 "
                          "refinement); default fp64")
@@ -104,6 +106,8 @@ def main():
         return bench_slices(args)
     if args.workload == "xxz":
         return bench_xxz(args)
+    if args.workload == "cheb":
+        return bench_cheb(args)
     args.L = args.L or 48
     args.seed = args.L if args.seed is None else args.seed
     args.s = args.s or 64
@@ -492,6 +496,71 @@ def bench_xxz(args):
         "detail": dict(n=n, nnz=dm.nnz, interval=[emin, emax], centre=e0, valid=nv,
                        pairs=int(len(lam)), t_filter=st["t_filter"], t_orth=st["t_orth"],
                        t_extract=st["t_extract"], t_refine=st.get("t_refine", 0.0),
+                       max_rel_res=float(res.max() / anorm) if len(res) else None),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_cheb(args):
+    """chebyshev_poly_real_symm_sparse on 3D Anderson L=20 (n=8,000), W=16.5,
+    seed 20, interval [-0.2, 0.2], FP64: m0 = 2 x the KPM count (the
+    reference harness's padding), degree 700, maxiter 5 (the sparse form runs
+    exactly maxiter iterations).  The CPU baseline is the oracle's full NumPy
+    restatement of the same call on a bounded sample (one iteration)."""
+    import sparse_matrix_algorithms_amd as sm
+    from sparse_matrix_algorithms_amd import kpm
+    from sparse_matrix_algorithms_amd.lattices import anderson3d
+    L = args.L or 20
+    seed = L if args.seed is None else args.seed
+    A = anderson3d(L, args.W, seed)
+    n = A.shape[0]
+    ctx, dm, t_up = _ctx_matrix(A)
+    smin, smax = dm.gershgorin_bounds()
+    anorm = max(abs(smin), abs(smax))
+    emin, emax = -0.2, 0.2
+    mu, _, _ = kpm.kpm_moments(dm, args.kpm_moments, args.kpm_probes, 7, smin, smax, ctx=ctx)
+    m0 = int(math.ceil(2.0 * kpm.count(mu, n, emin, emax, smin, smax)))
+    degree, maxiter = 700, 5
+    kw = dict(m0=m0, degree=degree, maxiter=maxiter, tol=1e-12, return_stats=True, ctx=ctx)
+    for w in range(args.warmup):
+        sm.chebyshev_poly_real_symm_sparse(dm, emin, emax, **kw)
+    times = []
+    for k in range(args.steps):
+        t = time.perf_counter()
+        lam, _, res, st = sm.chebyshev_poly_real_symm_sparse(dm, emin, emax, **kw)
+        times.append(time.perf_counter() - t)
+        log(f"[bench] step {k}: {times[-1]:.3f}s {st['line']} t_filter={st['t_filter']:.3f}")
+    dt = float(np.mean(times))
+    nv, _ = valid_pairs(lam, res, emin, emax, anorm)
+    roof = filter_roofline(dict(st, tile_vec=0, tile_lpr=0, tile_rt=0), "fp64", False)
+    roof["kernel"] = "clenshaw_kernel<double,…,false> (m0-wide panels)"
+    cpu = None
+    if not args.skip_cpu:
+        from oracle import cheb_ref
+        from sparse_matrix_algorithms_amd.lattices import gaussian_block
+        Q0 = gaussian_block(1, n, m0)
+        t = time.perf_counter()
+        cheb_ref.chebyshev_poly_real_symm_sparse(A, emin, emax, smin=smin, smax=smax, m0=m0,
+                                                 degree=degree, maxiter=1, tol=1e-12, Q0=Q0)
+        t1 = time.perf_counter() - t
+        t_cpu = t1 * maxiter
+        cpu = dict(value=round(nv / t_cpu, 4), unit="eigenpairs/s", cores=1, kind="port",
+                   sample=(f"oracle NumPy/SciPy chebyshev_poly_real_symm_sparse, n={n}, m0={m0}, "
+                           f"degree {degree}: one iteration {t1:.1f}s, x{maxiter} iterations "
+                           f"(SciPy SpMM is single-threaded)"))
+        log(f"[bench] cpu: {t1:.1f}s per iteration -> {t_cpu:.1f}s")
+    out = {
+        "metric": "eigenpairs/s at rel. resid <=1e-10, Chebyshev subspace iteration",
+        "value": round(nv / dt, 3), "unit": "eigenpairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+        "data": "synthetic (seeded 3D Anderson disorder)",
+        "config": {"workload": f"chebyshev_poly_real_symm_sparse, 3D Anderson L={L} W={args.W} "
+                               f"n={n}, [-0.2, 0.2], m0={m0}, degree {degree}, maxiter {maxiter}, "
+                               f"FP64", "L": L, "n": n, "m0": m0, "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu,
+        "detail": dict(n=n, nnz=dm.nnz, valid=nv, pairs=int(len(lam)), t_filter=st["t_filter"],
+                       t_orth=st["t_orth"], t_extract=st["t_extract"],
                        max_rel_res=float(res.max() / anorm) if len(res) else None),
     }
     print(json.dumps(out), flush=True)

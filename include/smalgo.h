@@ -202,7 +202,8 @@ typedef struct smalgo_polfed_params {
                             Ritz vectors left after the candidates) */
   int refine_degree;     /* degree of the FP64 refinement filter (default -1: degree) */
   int refine_maxiter;    /* FP64 refinement iterations with locking (default 3) */
-  double refine_tol;     /* relative residual target for locking (default 1e-10 * 0.5) */
+  double refine_tol;     /* lock a pair when ‖r‖ < refine_tol·‖A‖; default -1: when ‖r‖ < tol,
+                            the caller's absolute tolerance of the return rule */
 } smalgo_polfed_params;
 
 void smalgo_polfed_default_params(smalgo_polfed_params* p);

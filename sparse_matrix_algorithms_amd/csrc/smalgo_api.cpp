@@ -477,7 +477,7 @@ void smalgo_polfed_default_params(smalgo_polfed_params* p) {
   p->refine_guard = -1;
   p->refine_degree = -1;
   p->refine_maxiter = 3;
-  p->refine_tol = 0.5e-10;
+  p->refine_tol = -1.0;
 }
 
 void smalgo_cheb_default_params(smalgo_cheb_params* p) {

@@ -327,7 +327,11 @@ void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::
                   double* pbuf) {
   const char* nop = std::getenv("SMALGO_NO_PANELS");
   const bool panels_ok = !(nop && nop[0] == '1');
-  if (!op.is_callback() && panels_ok && b > cache_panel_width<T>(op)) {
+  // Panels only pay when one launch over the whole block would not stay
+  // cache-resident; small problems run the whole width in one launch per step.
+  const bool fits = !op.is_callback() &&
+                    3.0 * op.n * b * sizeof(T) + (double)op.A->nnz * (sizeof(T) + 4) <= 240e6;
+  if (!op.is_callback() && panels_ok && !fits && b > cache_panel_width<T>(op)) {
     // Wide blocks: repack into dense n x PW slabs (PW from the cache-aware
     // rule) so that every gather and stream of one launch chunk walks
     // contiguous rows, then unpack the result.
@@ -1468,7 +1472,8 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
         // ---- FP64 refinement of an FP32-built basis (no reference counterpart):
         // filtered subspace iteration on the candidates plus guard vectors (the
         // next Ritz vectors by θ), Rayleigh–Ritz in FP64, locking of pairs whose
-        // residual meets refine_tol·‖A‖ (‖A‖ = max(|smin|,|smax|)).
+        // residual is below refine_tol·‖A‖ (‖A‖ = max(|smin|,|smax|)) or, by
+        // default, below the caller's tol, the bound of the return rule.
         const size_t hr = tm.begin("refine");
         const int guard = p.refine_guard >= 0 ? p.refine_guard : nev;
         std::vector<int> zc(idxc);
@@ -1485,7 +1490,7 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
         const int Kr = p.refine_degree >= 0 ? p.refine_degree : degree;
         const std::vector<double> cr =
             build_coeffs(Kr, smin, smax, p.sigma, p.emin, p.emax, p.window_type, p.jackson != 0);
-        const double tol_abs = p.refine_tol * anorm;
+        const double tol_abs = p.refine_tol >= 0 ? p.refine_tol * anorm : p.tol;
         RRWork rr;
         int nlock = 0;
         std::vector<double> lam_lock, res_lock;
@@ -1529,7 +1534,7 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
           int inside_unconv = 0;
           for (int i = 0; i < kq; ++i) {
             const bool inside = p.emin <= lam_act[i] && lam_act[i] <= p.emax;
-            if (inside && res_act[i] <= tol_abs && nlock + (int)lock.size() < ldz)
+            if (inside && res_act[i] < tol_abs && nlock + (int)lock.size() < ldz)
               lock.push_back(i);
             else if (inside) {
               keep.push_back(i);

@@ -403,7 +403,8 @@ def _polfed_params(ctx, s, smin, smax, sigma, emin, emax, tol, overextract, refi
     p.precision = L.PRECISIONS[precision]
     p.refine_guard = -1 if refine_guard is None else refine_guard
     p.refine_degree = -1 if refine_degree is None else refine_degree
-    p.refine_maxiter, p.refine_tol = refine_maxiter, refine_tol
+    p.refine_maxiter = refine_maxiter
+    p.refine_tol = -1.0 if refine_tol is None else refine_tol
     return p
 
 
@@ -415,7 +416,7 @@ def polfed(A, nev: int, degree: int, s: int = 8, smin=None, smax=None, sigma: fl
            show_progress: bool = False, chunk: int = 64, Q1: Optional[np.ndarray] = None,
            precision: str = "fp64", refine_guard: Optional[int] = None,
            refine_degree: Optional[int] = None, refine_maxiter: int = 3,
-           refine_tol: float = 0.5e-10, return_stats: bool = False,
+           refine_tol: Optional[float] = None, return_stats: bool = False,
            ctx: Optional[Context] = None, want_X: bool = True):
     """``polfed(A, nev, degree; kwargs...)`` on the GPU.
 
@@ -522,7 +523,7 @@ def polfed_slices(A, slices, s: int = 8, smin=None, smax=None, tol: float = 1e-1
                   jackson: bool = True, rtol: float = 1e-12, reorth_passes: int = 1,
                   extra_keep: int = 8, seed: int = 1, precision: str = "fp64",
                   refine_guard: Optional[int] = None, refine_degree: Optional[int] = None,
-                  refine_maxiter: int = 3, refine_tol: float = 0.5e-10, use_graph: bool = True,
+                  refine_maxiter: int = 3, refine_tol: Optional[float] = None, use_graph: bool = True,
                   show_progress: bool = False, want_X: bool = True,
                   ctx: Optional[Context] = None):
     """POLFED over many intervals of one sparse matrix in one batched run.
