@@ -408,22 +408,50 @@ template void apply_filter<float>(Ctx&, const Op&, int, const float*, int,
 // the caller keeps the first r (the reference's rule, polfed.jl:679-684).
 // ---------------------------------------------------------------------------
 struct OrthWork {
-  DBuf<double> G, Rinv, gemm_work;
+  DBuf<double> G, Rinv, gemm_work, cn, red;
   std::vector<double> hG;
 };
 
 template <typename T>
 static int cholqr(Ctx& cx, Dense& dn, OrthWork& ow, int n, int b, const T* W, int ldw, T* Qout,
-                  int ldq, T* tmp, int ldt, double rtol, std::vector<double>* Rout) {
+                  int ldq, T* tmp, int ldt, double rtol, std::vector<double>* Rout,
+                  bool colscale = false) {
   const double u = std::is_same<T, float>::value ? 0x1p-24 : 0x1p-53;
   ow.G.ensure((size_t)b * b);
   ow.Rinv.ensure((size_t)b * b);
   ow.gemm_work.ensure(gemm_tn_work_elems(n, b, b));
   ow.hG.resize((size_t)b * b);
-  std::vector<double> Racc((size_t)b * b, 0.0), Rp((size_t)b * b);
-  for (int i = 0; i < b; ++i) Racc[(size_t)i * b + i] = 1.0;
+  // R = R_passes ... R_1 · D.  The full product is formed only when the
+  // caller wants R (block Lanczos); the rank rule needs only its diagonal,
+  // which is the product of the diagonals.
+  const bool full = Rout != nullptr;
+  std::vector<double> Racc(full ? (size_t)b * b : 0, 0.0), Rp((size_t)b * b), dacc(b, 1.0);
   const T* cur = W;
   int ldc = ldw;
+  if (colscale) {
+    // equilibrate: unit columns first (same span; columns of very different
+    // norms, as after a filter, would make the Gram matrix ill-conditioned)
+    std::vector<double> d(b), dinv(b);
+    ow.cn.ensure(b);
+    ow.red.ensure(colred_work_elems(n, b));
+    col_dots<T>(cx.stream, n, b, W, ldw, W, ldw, ow.cn.p, ow.red.p, ow.red.n);
+    SMALGO_HIP(hipMemcpyAsync(d.data(), ow.cn.p, sizeof(double) * b, hipMemcpyDeviceToHost,
+                              cx.stream));
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+    for (int i = 0; i < b; ++i) {
+      d[i] = d[i] > 0 ? std::sqrt(d[i]) : 1.0;
+      dinv[i] = 1.0 / d[i];
+      dacc[i] = d[i];
+    }
+    SMALGO_HIP(hipMemcpyAsync(ow.cn.p, dinv.data(), sizeof(double) * b, hipMemcpyHostToDevice,
+                              cx.stream));
+    copy_block<T, T>(cx.stream, n, b, W, ldw, tmp, ldt);
+    scale_cols<T>(cx.stream, n, b, tmp, ldt, ow.cn.p);
+    cur = tmp;
+    ldc = ldt;
+  }
+  if (full)
+    for (int i = 0; i < b; ++i) Racc[(size_t)i * b + i] = dacc[i];
   bool shifted = false;
   int passes = 2;
   for (int pass = 0; pass < passes; ++pass) {
@@ -459,24 +487,27 @@ static int cholqr(Ctx& cx, Dense& dn, OrthWork& ow, int n, int b, const T* W, in
     // dst = cur · R^{-1}; R^{-1} column-major -> transB
     gemm_nn<T, double, T>(cx.stream, n, b, b, 1.0, cur, ldc, ow.G.p, b, true, 0.0, dst, ldd);
     SMALGO_HIP(hipStreamSynchronize(cx.stream));
-    // Racc <- Rp · Racc   (both upper triangular, column-major)
-    std::vector<double> Rn((size_t)b * b, 0.0);
-    for (int j = 0; j < b; ++j)
-      for (int i = 0; i <= j; ++i) {
-        double s = 0.0;
-        for (int k = i; k <= j; ++k) s += Rp[(size_t)k * b + i] * Racc[(size_t)j * b + k];
-        Rn[(size_t)j * b + i] = s;
-      }
-    Racc.swap(Rn);
+    if (full) {
+      // Racc <- Rp · Racc   (both upper triangular, column-major)
+      std::vector<double> Rn((size_t)b * b, 0.0);
+      for (int j = 0; j < b; ++j)
+        for (int i = 0; i <= j; ++i) {
+          double s = 0.0;
+          for (int k = i; k <= j; ++k) s += Rp[(size_t)k * b + i] * Racc[(size_t)j * b + k];
+          Rn[(size_t)j * b + i] = s;
+        }
+      Racc.swap(Rn);
+    }
+    for (int i = 0; i < b; ++i) dacc[i] *= Rp[(size_t)i * b + i];
     cur = dst;
     ldc = ldd;
   }
   if (cur != Qout) copy_block<T, T>(cx.stream, n, b, cur, ldc, Qout, ldq);
   double r0 = 0.0;
-  for (int i = 0; i < b; ++i) r0 = std::max(r0, std::fabs(Racc[(size_t)i * b + i]));
+  for (int i = 0; i < b; ++i) r0 = std::max(r0, std::fabs(dacc[i]));
   int r = 0;
   if (r0 > 0)
-    for (int i = 0; i < b; ++i) r += std::fabs(Racc[(size_t)i * b + i]) > rtol * r0 ? 1 : 0;
+    for (int i = 0; i < b; ++i) r += std::fabs(dacc[i]) > rtol * r0 ? 1 : 0;
   if (Rout) *Rout = std::move(Racc);
   return r;
 }
@@ -565,7 +596,8 @@ static int refine_rr(Ctx& cx, Dense& dn, const Op& op, int n, int k, const doubl
   const int ld = round_up(std::max(k, 1), 4);
   w.Q.ensure((size_t)n * ld);
   w.tmp.ensure((size_t)n * ld);
-  int kq = cholqr<double>(cx, dn, w.ow, n, k, X, ldx, w.Q.p, ld, w.tmp.p, ld, rtol, nullptr);
+  int kq = cholqr<double>(cx, dn, w.ow, n, k, X, ldx, w.Q.p, ld, w.tmp.p, ld, rtol, nullptr,
+                          true);
   if (kq == 0) {
     lam.clear();
     res.clear();
