@@ -42,11 +42,13 @@ def instance(i: int, base: int = 2026):
                 target=int(rng.integers(40, 241)))
 
 
-def _solve(sm, dm, nev, K, mmax, emin, emax, smin, smax, anorm, ctx, precision="fp64"):
+def _solve(sm, dm, nev, K, mmax, emin, emax, smin, smax, anorm, ctx, precision="fp64",
+           refine_degree=None):
     t = time.perf_counter()
     lam, _, res, st = sm.polfed(dm, nev, K, s=S, smin=smin, smax=smax, emin=emin, emax=emax,
                                 sigma=0.5 * (emin + emax), mmax=mmax, tol=1e-10 * anorm, ctx=ctx,
-                                return_stats=True, want_X=False, precision=precision)
+                                return_stats=True, want_X=False, precision=precision,
+                                refine_degree=refine_degree)
     dt = time.perf_counter() - t
     ok = (lam >= emin) & (lam <= emax) & (res <= 1e-10 * anorm)
     return np.sort(lam[ok]), dt, st
@@ -155,7 +157,7 @@ def train(path: Path, epochs: int = 3000, lr: float = 3e-3, out: Path = None):
 
 
 def evaluate(path: Path, weights: Path = None, margin_kappa=None, margin_factor=None,
-             precision: str = "fp64", degree_scale: float = 1.0):
+             precision: str = "fp64", degree_scale: float = 1.0, refine_scale=None):
     """Held-out instances (256..319): learned parameters (with the stochastic
     count fallback) versus the plain rule solved on the GPU in FP64.  The
     learned run uses `precision` and the learned degree times `degree_scale`
@@ -179,8 +181,9 @@ def evaluate(path: Path, weights: Path = None, margin_kappa=None, margin_factor=
                                          r["smin"], r["smax"], S, margin_kappa, margin_factor)
         if src != "fallback":  # the scale is applied here, on the FP64 rule's degree
             K = max(1, math.ceil(degree_scale * K))
+        kr = None if refine_scale is None else max(1, math.ceil(refine_scale * K))
         lam, t, st = _solve(sm, dm, r["nev"], K, mmax, r["emin"], r["emax"], r["smin"],
-                            r["smax"], anorm, ctx, precision)
+                            r["smax"], anorm, ctx, precision, kr)
         K0, m0 = r["plain"]["degree"], r["plain"]["mmax"]
         lam0, t0, st0 = _solve(sm, dm, r["nev"], K0, m0, r["emin"], r["emax"], r["smin"],
                                r["smax"], anorm, ctx)
@@ -198,6 +201,7 @@ def evaluate(path: Path, weights: Path = None, margin_kappa=None, margin_factor=
         dm.free()
     both = [x for x in rows if x["plain_ok"]]
     summ = dict(instances=len(rows), precision=precision, degree_scale=degree_scale,
+                refine_scale=refine_scale,
                 success_learned=sum(x["ok"] for x in rows),
                 success_plain=sum(x["plain_ok"] for x in rows),
                 fallback_rate=sum(x["source"] == "fallback" for x in rows) / max(len(rows), 1),
@@ -226,6 +230,8 @@ def main():
                     help="eval: precision of the learned-parameter runs")
     ap.add_argument("--degree-scale", type=float, default=1.0,
                     help="eval: factor on the learned degree")
+    ap.add_argument("--refine-scale", type=float, default=None,
+                    help="eval: refinement degree as a factor of the basis degree")
     ap.add_argument("--redo-unconverged", type=Path, default=None,
                     help="regenerate the instances of this log whose plain run did not converge")
     a = ap.parse_args()
@@ -239,7 +245,8 @@ def main():
     elif a.cmd == "train":
         train(a.data, a.epochs, a.lr, a.out)
     else:
-        summ, rows = evaluate(a.data, precision=a.precision, degree_scale=a.degree_scale)
+        summ, rows = evaluate(a.data, precision=a.precision, degree_scale=a.degree_scale,
+                              refine_scale=a.refine_scale)
         if a.out:
             Path(a.out).write_text(json.dumps(dict(summary=summ, rows=rows), indent=1))
 
