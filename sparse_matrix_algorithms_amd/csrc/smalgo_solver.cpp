@@ -116,21 +116,71 @@ FilterTile choose_tile(size_t elem, int b, std::initializer_list<const void*> pt
 // ---------------------------------------------------------------------------
 // operator application
 // ---------------------------------------------------------------------------
+void Staging::ensure(size_t count, int chunks) {
+  if (!up) SMALGO_HIP(hipStreamCreateWithFlags(&up, hipStreamNonBlocking));
+  if (!done) SMALGO_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+  while ((int)ev.size() < chunks) {
+    hipEvent_t e = nullptr;
+    SMALGO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ev.push_back(e);
+  }
+  if (count <= cap) return;
+  if (x) (void)hipHostFree(x);
+  if (y) (void)hipHostFree(y);
+  x = y = nullptr;
+  cap = 0;
+  SMALGO_HIP(hipHostMalloc((void**)&x, sizeof(double) * count, hipHostMallocDefault));
+  SMALGO_HIP(hipHostMalloc((void**)&y, sizeof(double) * count, hipHostMallocDefault));
+  cap = count;
+}
+
+Staging::~Staging() {
+  if (x) (void)hipHostFree(x);
+  if (y) (void)hipHostFree(y);
+  for (auto e : ev) (void)hipEventDestroy(e);
+  if (done) (void)hipEventDestroy(done);
+  if (up) (void)hipStreamDestroy(up);
+}
+
+// Y = A X through the host callback.  The device block goes column-major,
+// then to pinned host memory in column chunks; the callback runs on chunk c
+// while later chunks are still in flight device->host, and each result chunk
+// goes back host->device on a second stream while the callback works on the
+// next one.  matblock!(Y, X) acts column by column, so calling it per chunk
+// is the same operator application.
 static void callback_apply(Ctx& cx, const Op& op, int b, const double* X, int ldx, double* Y,
                            int ldy) {
-  // device row-major -> host column-major -> user -> back
   const int n = op.n;
-  std::vector<double> hx((size_t)n * b), hy((size_t)n * b, 0.0);
-  DBuf<double> tmp((size_t)n * b);
-  rm_to_cm<double>(cx.stream, n, b, X, ldx, nullptr, tmp.p, n);
-  SMALGO_HIP(hipMemcpyAsync(hx.data(), tmp.p, sizeof(double) * hx.size(), hipMemcpyDeviceToHost,
-                            cx.stream));
-  SMALGO_HIP(hipStreamSynchronize(cx.stream));
-  const int rc = op.fn(op.user, n, b, hx.data(), n, hy.data(), n);
-  if (rc != 0) throw Error(-5, "matblock callback returned " + std::to_string(rc));
-  SMALGO_HIP(hipMemcpyAsync(tmp.p, hy.data(), sizeof(double) * hy.size(), hipMemcpyHostToDevice,
-                            cx.stream));
-  cm_to_rm<double, double>(cx.stream, n, b, tmp.p, n, Y, ldy);
+  if (b == 0 || n == 0) return;
+  // about four chunks, none under 4 MB
+  const int min_cols = std::max(1, (int)((4u << 20) / (sizeof(double) * (size_t)n)));
+  const int cw = std::max(min_cols, (b + 3) / 4);
+  const int nch = (b + cw - 1) / cw;
+  Staging& sg = cx.staging;
+  sg.ensure((size_t)n * b, nch);
+  DBuf<double> dx((size_t)n * b), dy((size_t)n * b);
+  rm_to_cm<double>(cx.stream, n, b, X, ldx, nullptr, dx.p, n);
+  for (int c = 0; c < nch; ++c) {
+    const size_t off = (size_t)c * cw * n, cnt = (size_t)std::min(cw, b - c * cw) * n;
+    SMALGO_HIP(hipMemcpyAsync(sg.x + off, dx.p + off, sizeof(double) * cnt,
+                              hipMemcpyDeviceToHost, cx.stream));
+    SMALGO_HIP(hipEventRecord(sg.ev[c], cx.stream));
+  }
+  for (int c = 0; c < nch; ++c) {
+    const int cb = std::min(cw, b - c * cw);
+    const size_t off = (size_t)c * cw * n, cnt = (size_t)cb * n;
+    SMALGO_HIP(hipEventSynchronize(sg.ev[c]));
+    const int rc = op.fn(op.user, n, cb, sg.x + off, n, sg.y + off, n);
+    if (rc != 0) {
+      (void)hipStreamSynchronize(sg.up);
+      throw Error(-5, "matblock callback returned " + std::to_string(rc));
+    }
+    SMALGO_HIP(hipMemcpyAsync(dy.p + off, sg.y + off, sizeof(double) * cnt,
+                              hipMemcpyHostToDevice, sg.up));
+  }
+  SMALGO_HIP(hipEventRecord(sg.done, sg.up));
+  SMALGO_HIP(hipStreamWaitEvent(cx.stream, sg.done, 0));
+  cm_to_rm<double, double>(cx.stream, n, b, dy.p, n, Y, ldy);
   SMALGO_HIP(hipStreamSynchronize(cx.stream));
 }
 

@@ -40,6 +40,19 @@ struct DBuf {
   ~DBuf() { release(); }
 };
 
+// Pinned host staging for host-callback operators: two grow-only pinned
+// blocks, a second stream for the host->device leg and per-chunk events.
+struct Staging {
+  double* x = nullptr;
+  double* y = nullptr;
+  size_t cap = 0;  // doubles per block
+  hipStream_t up = nullptr;
+  std::vector<hipEvent_t> ev;
+  hipEvent_t done = nullptr;
+  void ensure(size_t count, int chunks);
+  ~Staging();
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -49,6 +62,7 @@ struct Ctx {
   bool autotune = true;
   FilterTile last_tile{};  // tile of the most recent row-major filter
   std::map<long long, float> tuned_ms;  // per-step time of tuned per-column tiles
+  Staging staging;                      // host-callback operator transfers
 };
 
 struct Mat {
