@@ -23,7 +23,7 @@ using SparseArrays
 
 export polfed, polfed_slices, chebyshev_poly_real_symm_sparse, MatrixFreeRealSymOp, gershgorin_bounds
 
-const libsmalgo = get(ENV, "SMALGO_LIB", joinpath(@__DIR__, "..", "libsmalgo.so"))
+const libsmalgo = get(ENV, "SMALGO_LIB", joinpath(@__DIR__, "..", "..", "..", "libsmalgo.so"))
 
 const WINDOWS = Dict(:step => Cint(0), :delta => Cint(1), :smooth => Cint(2))
 
