@@ -193,7 +193,7 @@ def main():
                   kpm_seconds=round(t_k, 3), upload_seconds=round(t_up, 3),
                   headline=dict(precision=head_prec, **head))
     # dominant kernel: the fused Clenshaw step of the basis build
-    launches = st["filter_steps"]
+    launches = st.get("filter_launches", st["filter_steps"])
     avg_launch = st["t_filter"] / max(launches, 1)
     bytes_per_launch = st["spmm_bytes"] / max(launches, 1)
     achieved = bytes_per_launch / avg_launch / 1e9
@@ -296,7 +296,8 @@ def cpu_baseline(A, b, smin, smax, emin, emax, steps, pairs, seconds, what):
 def filter_roofline(st, prec, percol, bytes_key="spmm_bytes", launches=None):
     """Achieved algorithmic bandwidth of the fused Clenshaw kernel from the
     solver's event-timed filter phase and its modelled bytes."""
-    launches = launches if launches is not None else st["filter_steps"]
+    if launches is None:
+        launches = st.get("filter_launches", st["filter_steps"])
     avg_launch = st["t_filter"] / max(launches, 1)
     bpl = st[bytes_key] / max(launches, 1)
     achieved = bpl / avg_launch / 1e9
@@ -532,8 +533,7 @@ def bench_cheb(args):
         log(f"[bench] step {k}: {times[-1]:.3f}s {st['line']} t_filter={st['t_filter']:.3f}")
     dt = float(np.mean(times))
     nv, _ = valid_pairs(lam, res, emin, emax, anorm)
-    roof = filter_roofline(dict(st, tile_vec=0, tile_lpr=0, tile_rt=0), "fp64", False)
-    roof["kernel"] = "clenshaw_kernel<double,…,false> (m0-wide panels)"
+    roof = filter_roofline(st, "fp64", False)
     cpu = None
     if not args.skip_cpu:
         from oracle import cheb_ref

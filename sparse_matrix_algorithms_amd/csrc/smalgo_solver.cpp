@@ -399,6 +399,8 @@ void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::
     FilterTile tile{};
     if (!env_tile("SMALGO_PANEL_TILE", &tile))
       tile = autotune_tile<T>(cx, op, pw, Qp.p, pw, P0.p, P1.p, pw, Yp.p, pw);
+    cx.last_tile = tile;
+    cx.last_launches_per_step = npan;
     for (int pn = 0; pn < npan; ++pn) {
       const int bp = std::min(pw, b - pn * pw);
       const T* Qn = Qp.p + (size_t)pn * ps;
@@ -435,6 +437,7 @@ void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::
       tile = autotune_tile<T>(cx, op, b, Q, ldq, buf0, buf1, ldb, Y, ldy);
     tp = &tile;
     cx.last_tile = tile;
+    cx.last_launches_per_step = 1;
   }
   auto step = [&](double cq, double a1, double b1, FBlk<T> X1, FBlk<T> X2, T* out) {
     const int ldo = (out == Y) ? ldy : ldb;
@@ -1811,6 +1814,7 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
   fill_block<T>(cx.stream, n, B.ldV, B.V.p, B.ldV, T(0));
   basis_start<T>(cx, dn, sw.ow, B, n, Q1host, p.seed, p.rtol);
   FilterTile basis_tile{};
+  int basis_lps = 1;
   long long filter_steps = 0;
   double gemm_flops = 0;
   for (int j = 0; j < mmax && !B.done; ++j) {
@@ -1819,7 +1823,10 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
     apply_filter<T>(cx, op, B.r_cur, Qj, B.ldV, coeffs, smin, smax, W.p, lds, b0.p, b1.p, lds,
                     pbuf.p);
     tm.end(hf);
-    if (j == 0) basis_tile = cx.last_tile;
+    if (j == 0) {
+      basis_tile = cx.last_tile;
+      basis_lps = cx.last_launches_per_step;
+    }
     filter_steps += degree;  // K fused steps (the all-zero first step is skipped)
     size_t ho = tm.begin("orth");
     const int r = B.r_cur;
@@ -1837,6 +1844,7 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
   out->stats["t_extract"] = ph["extract"];
   out->stats["t_refine"] = ph["refine"];
   out->stats["filter_steps"] = (double)filter_steps;
+  out->stats["filter_launches"] = (double)filter_steps * basis_lps;
   out->stats["refine_filter_steps"] = (double)xs.refine_steps;
   out->stats["refine_iterations"] = xs.refine_iterations;
   out->stats["t_refine_host"] = xs.refine_host_s;
@@ -1939,10 +1947,14 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
   bool prev_ok = false;
   int its = 0;
   long long filter_steps = 0;
+  FilterTile cheb_tile{};
+  int cheb_lps = 1;
   for (int it = 1; it <= p.maxiter; ++it) {
     its = it;
     size_t hf = tm.begin("filter");
     apply_filter<double>(cx, op, m0, Q.p, ld, coeffs, smin, smax, Y.p, ld, B0.p, B1.p, ld, pbuf.p);
+    cheb_tile = cx.last_tile;
+    cheb_lps = cx.last_launches_per_step;
     tm.end(hf);
     filter_steps += p.degree;
     size_t ho = tm.begin("orth");
@@ -2016,6 +2028,11 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
   out->stats["t_extract"] = ph["rr"];
   out->stats["iterations"] = its;
   out->stats["filter_steps"] = (double)filter_steps;
+  out->stats["filter_launches"] = op.is_callback() ? (double)filter_steps
+                                                   : (double)filter_steps * cheb_lps;
+  out->stats["tile_vec"] = cheb_tile.vec;
+  out->stats["tile_lpr"] = cheb_tile.lpr;
+  out->stats["tile_rt"] = cheb_tile.rt;
   out->stats["smin"] = smin;
   out->stats["smax"] = smax;
   out->stats["spmm_bytes"] = (double)filter_steps * ((double)(op.A ? op.A->nnz : 0) * 12.0 +

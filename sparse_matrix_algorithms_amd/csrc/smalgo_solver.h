@@ -60,7 +60,8 @@ struct Ctx {
   std::string err;
   std::map<long long, FilterTile> tuned;  // (precision, b, matrix id, alignment) -> tile
   bool autotune = true;
-  FilterTile last_tile{};  // tile of the most recent row-major filter
+  FilterTile last_tile{};  // tile of the most recent filter
+  int last_launches_per_step = 1;  // kernel launches per filter step (panels)
   std::map<long long, float> tuned_ms;  // per-step time of tuned per-column tiles
   Staging staging;                      // host-callback operator transfers
 };

@@ -253,6 +253,7 @@ def apply_poly_block(A, Q: np.ndarray, coeffs: np.ndarray, smin: float, smax: fl
 # results
 # ---------------------------------------------------------------------------
 _STATS = ("t_total", "t_filter", "t_orth", "t_extract", "t_refine", "filter_steps",
+          "filter_launches",
           "block_steps", "basis_dim", "iterations", "spmm_bytes", "gemm_flops",
           "refine_filter_steps", "locked", "smin", "smax", "warn_interval", "tile_vec",
           "tile_lpr", "tile_rt", "slice_panel_width", "slice_replans", "slice_graphs",
