@@ -54,6 +54,8 @@ def parse():
                     help="slices: also run the first N slices one polfed call at a time and "
                          "as a batch of N")
     ap.add_argument("--eager", action="store_true", help="slices: launch without HIP graphs")
+    ap.add_argument("--slice-learned", action="store_true",
+                    help="slices: learned filter parameters per slice instead of the plain rule")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--L", type=int, default=None, help="lattice size (48 / 64 / 20 by workload)")
@@ -348,16 +350,26 @@ def bench_slices(args):
     anorm = max(abs(smin), abs(smax))
     mu, _, _ = kpm.kpm_moments(dm, args.kpm_moments, args.kpm_probes, 7, smin, smax, ctx=ctx)
     ns = args.slices
+    model = None
+    if args.slice_learned:
+        from sparse_matrix_algorithms_amd.learned import MLP, choose_parameters
+        model = MLP.load(ctx=ctx)
     slices = []
     for i in range(ns):
         c = -4.0 + 8.0 * (i + 0.5) / ns
         nev = SLICE_NEV[i % len(SLICE_NEV)]
         a, b = kpm.interval_for_count(mu, n, args.fill * nev, smin, smax, c)
         s = slice_width(nev)
-        K, mm = kpm.plain_parameters(n, nev, a, b, smin, smax, s, args.kappa, args.mmax_factor)
+        src = "plain"
+        if model is not None:
+            K, mm, src = choose_parameters(model, mu[:64], n, nev, a, b, smin, smax, s,
+                                           precision=prec)
+        else:
+            K, mm = kpm.plain_parameters(n, nev, a, b, smin, smax, s, args.kappa,
+                                         args.mmax_factor)
         slices.append(dict(emin=a, emax=b, sigma=c, nev=nev, s=s, degree=K, mmax=mm))
         log(f"[bench] slice {i}: centre {c:+.3f} [{a:.5f},{b:.5f}] nev={nev} s={s} K={K} "
-            f"mmax={mm}")
+            f"mmax={mm} ({src})")
     kw = dict(smin=smin, smax=smax, tol=1e-10 * anorm, precision=prec, ctx=ctx, want_X=False,
               refine_guard=args.refine_guard, refine_degree=args.refine_degree,
               refine_maxiter=args.refine_maxiter)
@@ -432,7 +444,9 @@ def bench_slices(args):
         "data": "synthetic (seeded 3D Anderson disorder)",
         "config": {"workload": f"batched POLFED slices, 3D Anderson L={L} W={args.W} n={n}, "
                                f"{ns} intervals centred in [-4,4] holding 50-400 pairs each, "
-                               f"block 8-64, {prec}", "L": L, "n": n, "slices": ns,
+                               f"block 8-64, {prec}, "
+                               f"{'learned' if model is not None else 'plain'} parameters",
+                   "L": L, "n": n, "slices": ns,
                    "pairs_requested": sum(d["nev"] for d in slices),
                    "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu, "detail": detail,
