@@ -1475,7 +1475,6 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
   long long refine_steps = 0;
   int locked_total = 0;
   double gemm_flops = 0;
-  double t_refine = 0;
   size_t hx = tm.begin("extract");
   Result* out = new Result();
   out->n = n;
@@ -1774,7 +1773,6 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
   xs.refine_steps += refine_steps;
   xs.locked += locked_total;
   xs.gemm_flops += gemm_flops;
-  (void)t_refine;
   return out;
 }
 
@@ -1829,9 +1827,7 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
     }
     filter_steps += degree;  // K fused steps (the all-zero first step is skipped)
     size_t ho = tm.begin("orth");
-    const int r = B.r_cur;
     lanczos_step<T>(cx, dn, sw, p, n, B, j, W.p, lds, b0.p, lds, Cbuf, gemm_flops);
-    (void)r;
     tm.end(ho);
   }
   ExtractStats xs;
