@@ -139,6 +139,18 @@ int smalgo_residuals(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn
 int smalgo_refine_original_rr(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn,
                               void* user, int64_t n, int64_t k, const double* X, int64_t ldx,
                               smalgo_result** out);
+/* The Rayleigh–Ritz tail of the reference's subspace iterations, real
+ * symmetric case: chebyshev_subspace_iteration.jl:733-748 and the same steps
+ * in feast.jl:540-553 and zoloterov.jl:1698-1706.  Q = thin_orth(Y) (all k
+ * columns), Aq = Qᵀ(A Q), eigen(Symmetric(Aq)), X = Q·Xq,
+ * res_j = ‖A x_j − λ_j x_j‖₂.  Y: host n x k column-major (the filtered or
+ * contour-integrated block).  The result holds all k pairs in ascending λ
+ * (eigen's order); the caller applies its own inside/res_gate selection.  Q
+ * (optional, host n x k) receives the orthonormal basis for the next
+ * iteration's solves. */
+int smalgo_subspace_rr(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn, void* user,
+                       int64_t n, int64_t k, const double* Y, int64_t ldy, double* Q,
+                       int64_t ldq, smalgo_result** out);
 
 /* ---- kernel polynomial method (no reference counterpart; sizes intervals and
  * feeds the learned filter parameters) ----------------------------------

@@ -12,7 +12,7 @@ from .solvers import (Context, DeviceMatrix, MatrixFreeRealSymOp, SmalgoError, a
                       cheb_window_coeffs, chebyshev_poly_real_symm_sparse,
                       chebyshev_poly_real_symm_sparse_mf, default_context, gershgorin_bounds,
                       jackson_kernel, polfed, polfed_mf)
-from .solvers import apply_poly_slices, polfed_slices
+from .solvers import apply_poly_slices, polfed_slices, subspace_rr
 
 __all__ = [
     "Context", "DeviceMatrix", "MatrixFreeRealSymOp", "SmalgoError", "apply_poly_block",
@@ -20,5 +20,5 @@ __all__ = [
     "chebyshev_poly_real_symm_sparse", "chebyshev_poly_real_symm_sparse_mf", "default_context",
     "gershgorin_bounds", "jackson_kernel", "polfed", "polfed_mf", "block_qr_factor", "build_X",
     "orth_against", "orth_block_polfed", "rayleigh", "refine_original_rr", "residuals",
-    "apply_poly_slices", "polfed_slices",
+    "apply_poly_slices", "polfed_slices", "subspace_rr",
 ]

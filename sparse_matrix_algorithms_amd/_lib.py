@@ -110,6 +110,9 @@ SIGNATURES = [
     ("smalgo_polfed", C.c_int,
      [_P, _P, MATBLOCK_FN, _P, C.c_int64, C.c_int, C.c_int, C.POINTER(PolfedParams), _DP,
       C.POINTER(_P)]),
+    ("smalgo_subspace_rr", C.c_int,
+     [_P, _P, MATBLOCK_FN, _P, C.c_int64, C.c_int64, _DP, C.c_int64, _DP, C.c_int64,
+      C.POINTER(_P)]),
     ("smalgo_polfed_slices", C.c_int,
      [_P, _P, C.c_int, C.POINTER(SliceParams), C.POINTER(PolfedParams), C.c_int,
       C.POINTER(_P)]),

@@ -362,6 +362,19 @@ int smalgo_refine_original_rr(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matbl
   });
 }
 
+int smalgo_subspace_rr(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn, void* user,
+                       int64_t n, int64_t k, const double* Y, int64_t ldy, double* Q,
+                       int64_t ldq, smalgo_result** out) {
+  if (!ctx || !out) return SMALGO_ERR_ARG;
+  *out = nullptr;
+  return guarded(ctx, [&] {
+    Op op;
+    if (!make_op(A, fn, user, n, op)) throw Error(-1, "exactly one of A or fn must be given (with matching n)");
+    if (k < 1 || k > op.n || ldy < op.n || !Y || (Q && ldq < op.n)) throw Error(-1, "bad arguments");
+    *out = handle_of(subspace_rr_host(*ctx, op, (int)k, Y, (int)ldy, Q, (int)ldq));
+  });
+}
+
 int smalgo_kpm_moments(smalgo_ctx* ctx, const smalgo_mat* A, int nmom, int R, uint64_t seed,
                        double smin, double smax, double* mu) {
   if (!ctx || !A || !mu) return SMALGO_ERR_ARG;

@@ -787,6 +787,55 @@ void residuals_host(Ctx& cx, const Op& op, int k, const double* X, int ldx, cons
   SMALGO_HIP(hipStreamSynchronize(cx.stream));
 }
 
+// The Rayleigh–Ritz tail of the reference's subspace iterations
+// (chebyshev_subspace_iteration.jl:733-748, feast.jl:540-553,
+// zoloterov.jl:1698-1706, real symmetric case) on a host block Y:
+// Q = thin_orth(Y) keeping all k columns, Aq = Qᵀ(A Q), eigen(Symmetric(Aq)),
+// X = Q Xq, res_j = ‖A x_j - λ_j x_j‖₂.  All k pairs in ascending λ.
+Result* subspace_rr_host(Ctx& cx, const Op& op, int k, const double* Y, int ldy, double* Qh,
+                         int ldqh) {
+  const int n = op.n;
+  Result* out = new Result();
+  out->n = n;
+  out->device = cx.device;
+  try {
+    if (k == 0) return out;
+    const int ld = round_up(k, 4);
+    Dense dn(cx);
+    OrthWork ow;
+    DBuf<double> Yd((size_t)n * ld), Q((size_t)n * ld), T((size_t)n * ld), AX((size_t)n * ld),
+        X((size_t)n * ld), Aq((size_t)k * k), wv(k), ev(k), resd(k);
+    upload_cm<double>(cx, n, k, Y, ldy, Yd.p, ld);
+    cholqr<double>(cx, dn, ow, n, k, Yd.p, ld, Q.p, ld, T.p, ld, 0.0, nullptr, true);
+    apply_A<double>(cx, op, k, Q.p, ld, AX.p, ld);
+    DBuf<double> gw(gemm_tn_work_elems(n, k, k));
+    gemm_tn<double, double, double>(cx.stream, n, k, k, 1.0, Q.p, ld, AX.p, ld, 0.0, Aq.p, k, gw.p,
+                                    gw.n);
+    // Symmetric(Aq) reads the upper triangle; Aq is row-major, i.e. the lower
+    // triangle of its column-major view
+    dn.syevd(k, Aq.p, k, wv.p, ev.p, false);
+    gemm_nn<double, double, double>(cx.stream, n, k, k, 1.0, Q.p, ld, Aq.p, k, true, 0.0, X.p, ld);
+    apply_A<double>(cx, op, k, X.p, ld, AX.p, ld);
+    DBuf<double> red(colred_work_elems(n, k));
+    col_resid<double>(cx.stream, n, k, X.p, ld, AX.p, ld, wv.p, resd.p, red.p, red.n);
+    out->lam.resize(k);
+    out->res.resize(k);
+    SMALGO_HIP(hipMemcpyAsync(out->lam.data(), wv.p, sizeof(double) * k, hipMemcpyDeviceToHost,
+                              cx.stream));
+    SMALGO_HIP(hipMemcpyAsync(out->res.data(), resd.p, sizeof(double) * k,
+                              hipMemcpyDeviceToHost, cx.stream));
+    SMALGO_HIP(hipMalloc((void**)&out->Xdev, sizeof(double) * (size_t)n * k));
+    rm_to_cm<double>(cx.stream, n, k, X.p, ld, nullptr, out->Xdev, n);
+    if (Qh) download_cm<double>(cx, n, k, Q.p, ld, Qh, ldqh);
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+    out->line = "subspace_rr k=" + std::to_string(k);
+  } catch (...) {
+    delete out;
+    throw;
+  }
+  return out;
+}
+
 Result* refine_rr_host(Ctx& cx, const Op& op, int k, const double* X, int ldx) {
   const int n = op.n, lk = round_up(std::max(k, 1), 4);
   DBuf<double> dX((size_t)n * lk);
@@ -1955,7 +2004,7 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
     filter_steps += p.degree;
     size_t ho = tm.begin("orth");
     // Q .= Y; thin_orth!(Q)  — all min(n,m0) columns kept (:733-734)
-    cholqr<double>(cx, dn, ow, n, m0, Y.p, ld, Q.p, ld, B0.p, ld, 0.0, nullptr);
+    cholqr<double>(cx, dn, ow, n, m0, Y.p, ld, Q.p, ld, B0.p, ld, 0.0, nullptr, true);
     tm.end(ho);
     size_t hr = tm.begin("rr");
     apply_A<double>(cx, op, m0, Q.p, ld, AX.p, ld);                       // R = A Q

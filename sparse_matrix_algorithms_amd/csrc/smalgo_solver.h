@@ -146,6 +146,8 @@ void rayleigh_host(Ctx& cx, const Op& op, int k, const double* X, int ldx, doubl
 void residuals_host(Ctx& cx, const Op& op, int k, const double* X, int ldx, const double* lam,
                     double* res);
 Result* refine_rr_host(Ctx& cx, const Op& op, int k, const double* X, int ldx);
+Result* subspace_rr_host(Ctx& cx, const Op& op, int k, const double* Y, int ldy, double* Qh,
+                         int ldqh);
 
 // one slice of a batched POLFED run: interval, window centre, pairs wanted,
 // block width, filter degree, basis blocks (<= 0: ceil(2.8 nev / s))

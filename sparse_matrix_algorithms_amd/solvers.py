@@ -464,6 +464,30 @@ def polfed_mf(matvec: Callable, n: int, nev: int, degree: int, matblock: Optiona
     return polfed(MatrixFreeRealSymOp(n, matvec, matblock), nev, degree, **kwargs)
 
 
+def subspace_rr(A, Y: np.ndarray, return_Q: bool = False, ctx: Optional[Context] = None):
+    """The Rayleigh–Ritz tail of the subspace iterations (Chebyshev ``:733-748``,
+    FEAST ``feast.jl:540-553``, Zolotarev ``zoloterov.jl:1698-1706``, real
+    symmetric case) on the GPU: returns ``(λ, X, res)`` for all columns in
+    ascending λ, plus ``Q = thin_orth(Y)`` with ``return_Q``."""
+    ctx = ctx or default_context()
+    dm, mf = _as_operator(A, ctx)
+    Yf = np.asfortranarray(Y, dtype=np.float64)
+    n, k = Yf.shape
+    Q = np.zeros((n, k), order="F") if return_Q else None
+    h = C.c_void_p()
+    if mf is not None:
+        rc = ctx.lib.smalgo_subspace_rr(ctx.h, None, mf.cfunc(), None, n, k, _dptr(Yf), n,
+                                        _dptr(Q) if return_Q else None, n, C.byref(h))
+        if mf._err:
+            raise mf._err.pop()
+    else:
+        rc = ctx.lib.smalgo_subspace_rr(ctx.h, dm.h, L.MATBLOCK_FN(), None, n, k, _dptr(Yf), n,
+                                        _dptr(Q) if return_Q else None, n, C.byref(h))
+    ctx.check(rc)
+    lam, X, res, _ = _take_result(ctx, h)
+    return (lam, X, res, Q) if return_Q else (lam, X, res)
+
+
 # ---------------------------------------------------------------------------
 # batched spectral slices (no reference counterpart)
 # ---------------------------------------------------------------------------
