@@ -298,6 +298,31 @@ int smalgo_chebyshev(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn
                      int64_t n, double emin, double emax, const smalgo_cheb_params* p,
                      const double* Q0, smalgo_result** out);
 
+/* ---- on-disk fixtures (no reference counterpart) --------------------------
+ * One little-endian binary format for matrices and eigenvalue sets, shared
+ * by the C, Python (sparse_matrix_algorithms_amd.fixtures) and Julia front
+ * ends.  64-byte header, then the payload; the header's checksum is
+ * Σ_i (w_i xor i·0x9E3779B97F4A7C15)·0xBF58476D1CE4E5B9 mod 2^64 over the
+ * payload's 8-byte words w_i.
+ *   CSR: "SMALGO-CSR-v1" (16 bytes, NUL-padded) | int64 n | int64 nnz |
+ *        int32 index_base = 0 | int32 flags (bit 0 symmetric) | uint64 checksum |
+ *        16 zero bytes | int64 rowptr[n+1] | int64 col[nnz] | float64 val[nnz]
+ *   Eig: "SMALGO-EIG-v1" | int64 count | int64 has_res | uint64 checksum |
+ *        24 zero bytes | float64 lam[count] | float64 res[count] if has_res
+ * These calls need no context and no GPU; smalgo_io_last_error() gives the
+ * message of the calling thread's last failure. */
+int smalgo_csr_save(const char* path, int64_t n, const int64_t* rowptr, const int64_t* colidx,
+                    const double* val, int index_base, int symmetric);
+int smalgo_csr_info(const char* path, int64_t* n, int64_t* nnz, int* symmetric);
+/* rowptr (n+1), colidx (nnz), val (nnz), 0-based; any may be NULL to skip */
+int smalgo_csr_load(const char* path, int64_t* rowptr, int64_t* colidx, double* val);
+/* load a CSR fixture and upload it (smalgo_csr_upload with index_base 0) */
+int smalgo_csr_upload_file(smalgo_ctx* ctx, const char* path, smalgo_mat** out);
+int smalgo_eig_save(const char* path, int64_t count, const double* lam, const double* res);
+/* lam and res NULL: only count and has_res */
+int smalgo_eig_load(const char* path, int64_t* count, int* has_res, double* lam, double* res);
+const char* smalgo_io_last_error(void);
+
 /* ---- results ---------------------------------------------------------------
  * (λ::Vector, X::Matrix n x k column-major, res::Vector) as returned by the
  * drivers (polfed.jl:1148-1157, chebyshev_subspace_iteration.jl:750-756). */

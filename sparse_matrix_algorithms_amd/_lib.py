@@ -113,6 +113,14 @@ SIGNATURES = [
     ("smalgo_subspace_rr", C.c_int,
      [_P, _P, MATBLOCK_FN, _P, C.c_int64, C.c_int64, _DP, C.c_int64, _DP, C.c_int64,
       C.POINTER(_P)]),
+    ("smalgo_csr_save", C.c_int,
+     [C.c_char_p, C.c_int64, _I64P, _I64P, _DP, C.c_int, C.c_int]),
+    ("smalgo_csr_info", C.c_int, [C.c_char_p, _I64P, _I64P, C.POINTER(C.c_int)]),
+    ("smalgo_csr_load", C.c_int, [C.c_char_p, _I64P, _I64P, _DP]),
+    ("smalgo_csr_upload_file", C.c_int, [_P, C.c_char_p, C.POINTER(_P)]),
+    ("smalgo_eig_save", C.c_int, [C.c_char_p, C.c_int64, _DP, _DP]),
+    ("smalgo_eig_load", C.c_int, [C.c_char_p, _I64P, C.POINTER(C.c_int), _DP, _DP]),
+    ("smalgo_io_last_error", C.c_char_p, []),
     ("smalgo_polfed_slices", C.c_int,
      [_P, _P, C.c_int, C.POINTER(SliceParams), C.POINTER(PolfedParams), C.c_int,
       C.POINTER(_P)]),

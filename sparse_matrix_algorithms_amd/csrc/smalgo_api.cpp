@@ -38,6 +38,21 @@ int guarded(smalgo_ctx* ctx, F&& f) {
     return SMALGO_ERR_INTERNAL;
   }
 }
+thread_local std::string g_io_err;
+template <typename F>
+int io_guarded(F&& f) {
+  try {
+    f();
+    g_io_err.clear();
+    return SMALGO_OK;
+  } catch (const Error& e) {
+    g_io_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_io_err = e.what();
+    return SMALGO_ERR_INTERNAL;
+  }
+}
 }  // namespace
 
 extern "C" {
@@ -608,6 +623,55 @@ int smalgo_chebyshev(smalgo_ctx* ctx, const smalgo_mat* A, smalgo_matblock_fn fn
     Result* r = chebyshev_solve(*ctx, op, emin, emax, prm, Q0, fn != nullptr);
     *out = handle_of(r);
   });
+}
+
+// ---- on-disk fixtures ------------------------------------------------------
+const char* smalgo_io_last_error(void) { return g_io_err.c_str(); }
+
+int smalgo_csr_save(const char* path, int64_t n, const int64_t* rowptr, const int64_t* colidx,
+                    const double* val, int index_base, int symmetric) {
+  if (!path) return SMALGO_ERR_ARG;
+  return io_guarded([&] { csr_save(path, n, rowptr, colidx, val, index_base, symmetric); });
+}
+
+int smalgo_csr_info(const char* path, int64_t* n, int64_t* nnz, int* symmetric) {
+  if (!path) return SMALGO_ERR_ARG;
+  return io_guarded([&] { csr_info(path, n, nnz, symmetric); });
+}
+
+int smalgo_csr_load(const char* path, int64_t* rowptr, int64_t* colidx, double* val) {
+  if (!path) return SMALGO_ERR_ARG;
+  return io_guarded([&] { csr_load(path, rowptr, colidx, val); });
+}
+
+int smalgo_csr_upload_file(smalgo_ctx* ctx, const char* path, smalgo_mat** out) {
+  if (!ctx || !path || !out) return SMALGO_ERR_ARG;
+  *out = nullptr;
+  int64_t n = 0, nnz = 0;
+  int sym = 0;
+  int rc = smalgo_csr_info(path, &n, &nnz, &sym);
+  if (rc != SMALGO_OK) {
+    ctx->err = g_io_err;
+    return rc;
+  }
+  std::vector<int64_t> rp(n + 1), ci(nnz);
+  std::vector<double> v(nnz);
+  rc = smalgo_csr_load(path, rp.data(), ci.data(), v.data());
+  if (rc != SMALGO_OK) {
+    ctx->err = g_io_err;
+    return rc;
+  }
+  return smalgo_csr_upload(ctx, n, rp.data(), ci.data(), v.data(), 0, out);
+}
+
+int smalgo_eig_save(const char* path, int64_t count, const double* lam, const double* res) {
+  if (!path) return SMALGO_ERR_ARG;
+  return io_guarded([&] { eig_save(path, count, lam, res); });
+}
+
+int smalgo_eig_load(const char* path, int64_t* count, int* has_res, double* lam, double* res) {
+  if (!path) return SMALGO_ERR_ARG;
+  return io_guarded([&] { eig_load(path, count, has_res, lam, res); });
 }
 
 // ---- results --------------------------------------------------------------

@@ -139,4 +139,12 @@ void stream_copy(hipStream_t st, const void* src, void* dst, size_t bytes);
 // number of compute units, queried once per device
 int device_cu_count();
 
+// on-disk fixtures (smalgo_io.cpp)
+void csr_save(const char* path, int64_t n, const int64_t* rowptr, const int64_t* col,
+              const double* val, int index_base, int symmetric);
+void csr_info(const char* path, int64_t* n, int64_t* nnz, int* symmetric);
+void csr_load(const char* path, int64_t* rowptr, int64_t* col, double* val);
+void eig_save(const char* path, int64_t count, const double* lam, const double* res);
+void eig_load(const char* path, int64_t* count, int* has_res, double* lam, double* res);
+
 }  // namespace smalgo

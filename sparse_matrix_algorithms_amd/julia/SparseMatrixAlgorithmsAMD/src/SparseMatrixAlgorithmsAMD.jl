@@ -22,6 +22,7 @@ using Random
 using SparseArrays
 
 export polfed, polfed_slices, chebyshev_poly_real_symm_sparse, MatrixFreeRealSymOp, gershgorin_bounds
+export save_csr, load_csr
 
 const libsmalgo = get(ENV, "SMALGO_LIB", joinpath(@__DIR__, "..", "..", "..", "libsmalgo.so"))
 
@@ -227,6 +228,32 @@ function polfed_slices(A::SparseMatrixCSC{Float64,Int}, slices; s::Int=8, smin=n
                context().h, dev.h, length(sl), sl, p, use_graph, hs)
     check(rc)
     return [take_result(h) for h in hs]
+end
+
+# ---- on-disk fixtures (the format of include/smalgo.h, shared with Python) ----------
+function ioerr(rc)
+    rc == 0 && return nothing
+    error(unsafe_string(ccall((:smalgo_io_last_error, libsmalgo), Cstring, ())))
+end
+
+"Write A (square) as a CSR fixture file; the CSC arrays of transpose(A) are A's CSR arrays."
+function save_csr(path::AbstractString, A::SparseMatrixCSC{Float64,Int})
+    T = SparseMatrixCSC(transpose(A))
+    ioerr(ccall((:smalgo_csr_save, libsmalgo), Cint,
+                (Cstring, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cdouble}, Cint, Cint),
+                path, size(A, 1), T.colptr, T.rowval, T.nzval, 1, issymmetric(A)))
+end
+
+"Read a CSR fixture file into a SparseMatrixCSC."
+function load_csr(path::AbstractString)
+    n = Ref{Int64}(0); nnz = Ref{Int64}(0); sym = Ref{Cint}(0)
+    ioerr(ccall((:smalgo_csr_info, libsmalgo), Cint, (Cstring, Ref{Int64}, Ref{Int64}, Ref{Cint}),
+                path, n, nnz, sym))
+    rp = Vector{Int64}(undef, n[] + 1); ci = Vector{Int64}(undef, nnz[]); v = Vector{Float64}(undef, nnz[])
+    ioerr(ccall((:smalgo_csr_load, libsmalgo), Cint, (Cstring, Ptr{Int64}, Ptr{Int64}, Ptr{Cdouble}),
+                path, rp, ci, v))
+    # CSR of A read as CSC is transpose(A); 0-based -> 1-based
+    return SparseMatrixCSC(transpose(SparseMatrixCSC(n[], n[], rp .+ 1, ci .+ 1, v)))
 end
 
 # ---- Chebyshev subspace iteration -------------------------------------------------
