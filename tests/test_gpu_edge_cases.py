@@ -1,0 +1,99 @@
+"""Edge cases against the oracle's reading of the reference: an interval with
+no eigenvalue returns empty results (polfed.jl:1156,
+chebyshev_subspace_iteration.jl:754), a rank-deficient start block keeps its
+first r columns (polfed.jl:561-594), degree 0 runs."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import polfed_ref as P
+from sparse_matrix_algorithms_amd.lattices import (anderson3d, gaussian_block, lap1d,
+                                                   lap1d_exact_eigs)
+
+pytestmark = pytest.mark.gpu
+
+
+def _gap_window():
+    n = 1000
+    lex = lap1d_exact_eigs(n)
+    k = 500
+    g = lex[k + 1] - lex[k]
+    return n, lex[k] + 0.25 * g, lex[k + 1] - 0.25 * g
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_polfed_interval_without_eigenvalues(ctx, precision):
+    import sparse_matrix_algorithms_amd as sm
+    n, emin, emax = _gap_window()
+    kw = dict(s=8, smin=0.0, smax=4.0, emin=emin, emax=emax, sigma=0.5 * (emin + emax), mmax=4)
+    Q1 = gaussian_block(1, n, 8)
+    lam, X, res = sm.polfed(lap1d(n), 10, 200, Q1=Q1, precision=precision, ctx=ctx, **kw)
+    lo, _, _ = P.polfed(lap1d(n), 10, 200, Q0=Q1, **kw)
+    assert len(lo) == 0
+    assert len(lam) == 0 and X.shape == (n, 0) and len(res) == 0
+
+
+def test_chebyshev_interval_without_eigenvalues(ctx):
+    import sparse_matrix_algorithms_amd as sm
+    n, emin, emax = _gap_window()
+    lam, X, res = sm.chebyshev_poly_real_symm_sparse(lap1d(n), emin, emax, smin=0.0, smax=4.0,
+                                                     m0=16, degree=100, maxiter=3, ctx=ctx)
+    assert len(lam) == 0 and X.shape == (n, 0) and len(res) == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_polfed_rank_deficient_start_block(ctx, precision):
+    """A start block of rank 5 in 8 columns: orth_block_polfed! keeps the first
+    5 columns (polfed.jl:561-594), every later block has width 5, and the run
+    still converges to the dense eigenvalues, as the oracle does."""
+    import sparse_matrix_algorithms_amd as sm
+    A = anderson3d(10, 16.5, 21)
+    n = A.shape[0]
+    ev = np.linalg.eigh(A.toarray())[0]
+    i0 = np.searchsorted(ev, 0.0) - 10
+    emin, emax = 0.5 * (ev[i0 - 1] + ev[i0]), 0.5 * (ev[i0 + 19] + ev[i0 + 20])
+    exact = ev[i0:i0 + 20]
+    smin, smax = P.gershgorin_bounds(A)
+    anorm = max(abs(smin), abs(smax))
+    Q1 = gaussian_block(4, n, 8)
+    Q1[:, 5] = Q1[:, 0] + 2.0 * Q1[:, 1]
+    Q1[:, 6] = Q1[:, 2] - Q1[:, 3]
+    Q1[:, 7] = 0.5 * Q1[:, 4]
+    kw = dict(s=8, smin=smin, smax=smax, emin=emin, emax=emax, mmax=16)
+    lam, X, res, st = sm.polfed(A, 20, 700, Q1=Q1, precision=precision, return_stats=True,
+                                ctx=ctx, **kw)
+    lo, _, ro = P.polfed(A, 20, 700, Q0=Q1, **kw)
+    assert len(lo) == 20 and np.max(np.abs(np.sort(lo) - exact)) <= 1e-10 * anorm
+    assert st["basis_dim"] == 5 * st["block_steps"]
+    assert len(lam) == 20
+    assert np.max(np.abs(np.sort(lam) - exact)) <= 1e-10 * anorm
+    assert res.max() <= 1e-10 * anorm
+
+
+def test_polfed_degree_zero_matches_oracle_count(ctx):
+    import sparse_matrix_algorithms_amd as sm
+    B = anderson3d(4, 16.5, 3)
+    n = B.shape[0]
+    smin, smax = P.gershgorin_bounds(B)
+    Q1 = gaussian_block(2, n, 8)
+    kw = dict(s=8, smin=smin, smax=smax, emin=-2.0, emax=2.0, mmax=4)
+    lam, X, res = sm.polfed(B, 5, 0, Q1=Q1, ctx=ctx, **kw)
+    lo, _, ro = P.polfed(B, 5, 0, Q0=Q1, **kw)
+    assert len(lam) == len(lo) == 5
+    assert np.all(np.isfinite(lam)) and np.all(np.isfinite(res))
+
+
+def test_polfed_slices_with_an_empty_slice(ctx):
+    import sparse_matrix_algorithms_amd as sm
+    n, gmin, gmax = _gap_window()
+    lex = lap1d_exact_eigs(n)
+    i0 = 200
+    emin, emax = 0.5 * (lex[i0 - 1] + lex[i0]), 0.5 * (lex[i0 + 19] + lex[i0 + 20])
+    slices = [dict(emin=gmin, emax=gmax, nev=6, s=8, degree=200, mmax=4),
+              dict(emin=emin, emax=emax, nev=20, s=8, degree=300, mmax=8)]
+    outs = sm.polfed_slices(lap1d(n), slices, smin=0.0, smax=4.0, ctx=ctx)
+    assert len(outs[0][0]) == 0
+    lam, _, res, _ = outs[1]
+    assert len(lam) == 20
+    assert np.max(np.abs(np.sort(lam) - lex[i0:i0 + 20])) <= 1e-10 * 4
