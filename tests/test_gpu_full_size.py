@@ -1,0 +1,130 @@
+"""Parity at the benchmark sizes: C2 (Anderson L=48, n=110,592), C3 (XXZ L=20,
+n=184,756) and C4 (Anderson L=64, n=262,144).
+
+The filter (``apply_poly_block!``, polfed.jl:464-545) is compared with the
+oracle's OpenMP FP64 Clenshaw (``oracle/clenshaw.c``, pinned to the NumPy
+restatement in ``test_oracle_filter.py``) at moderate degree.  The cases
+cover the whole-width launch, the cache-aware column panels (blocks whose
+three streamed n×b arrays exceed the cache budget) and the packed
+multi-slice filter at full size.
+
+The full C2 solve (``polfed``, polfed.jl:1128-1158) is checked through
+properties that hold at any size:
+- FP64 residuals and orthonormality of the returned vectors, recomputed on
+  the host;
+- the same eigenvalue set from the mixed and the FP64 paths;
+- the pair count against the stochastic KPM estimate of the interval's
+  eigenvalue count.
+"""
+import numpy as np
+import pytest
+
+from oracle import clenshaw_c
+from oracle import polfed_ref as P
+from sparse_matrix_algorithms_amd.lattices import anderson3d, gaussian_block, xxz_sz0
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c2():
+    A = anderson3d(48, 16.5, 48)
+    smin, smax = P.gershgorin_bounds(A)
+    return A, smin, smax
+
+
+@pytest.mark.parametrize("b,K", [(64, 200), (128, 100)])
+def test_c2_filter_fp64_vs_c_oracle(ctx, c2, b, K):
+    """b=64: one launch over the whole width; b=128 FP64 (340 MB of streamed
+    blocks): split into cache-aware panels."""
+    import sparse_matrix_algorithms_amd as sm
+    A, smin, smax = c2
+    Q = gaussian_block(21, A.shape[0], b)
+    co = P.build_coeffs(K, smin, smax, 0.0, -0.04, 0.04, "step")
+    Y = sm.apply_poly_block(A, Q, co, smin, smax, ctx=ctx)
+    Yo = clenshaw_c.apply_poly_block(A, Q, co, smin, smax)
+    err = np.max(np.abs(Y - Yo)) / np.max(np.abs(Yo))
+    assert err <= 1e-12, err
+
+
+def test_c2_filter_fp32_vs_c_oracle(ctx, c2):
+    """FP32 filter at C2: FP32 rounding only (1e-4 of the block norm)."""
+    import sparse_matrix_algorithms_amd as sm
+    A, smin, smax = c2
+    Q = gaussian_block(22, A.shape[0], 64)
+    co = P.build_coeffs(200, smin, smax, 0.0, -0.04, 0.04, "step")
+    Y = sm.apply_poly_block(A, Q, co, smin, smax, precision="fp32", ctx=ctx)
+    Yo = clenshaw_c.apply_poly_block(A, Q, co, smin, smax)
+    err = np.linalg.norm(Y - Yo) / np.linalg.norm(Yo)
+    assert err <= 1e-4, err
+
+
+def test_c3_filter_fp64_vs_c_oracle(ctx):
+    """XXZ rows hold 1..11 nonzeros; b=128 FP64 runs in panels at this n."""
+    import sparse_matrix_algorithms_amd as sm
+    A = xxz_sz0(20, 5.0, 20)
+    smin, smax = P.gershgorin_bounds(A)
+    Q = gaussian_block(23, A.shape[0], 128)
+    co = P.build_coeffs(40, smin, smax, 0.0, -0.3, 0.3, "step")
+    Y = sm.apply_poly_block(A, Q, co, smin, smax, ctx=ctx)
+    Yo = clenshaw_c.apply_poly_block(A, Q, co, smin, smax)
+    err = np.max(np.abs(Y - Yo)) / np.max(np.abs(Yo))
+    assert err <= 1e-12, err
+
+
+def test_c4_packed_slices_vs_c_oracle(ctx):
+    """Packed filter over four slices of different width and degree at
+    n=262,144 (one of them degree 0), each against the oracle."""
+    import sparse_matrix_algorithms_amd as sm
+    A = anderson3d(64, 16.5, 64)
+    smin, smax = P.gershgorin_bounds(A)
+    widths, degrees = [8, 24, 64, 40], [60, 30, 90, 0]
+    wins = [(-3.26, -3.24), (-0.26, -0.24), (0.74, 0.76), (2.24, 2.26)]
+    co = [P.build_coeffs(K, smin, smax, 0.5 * (a + b), a, b, "step")
+          for K, (a, b) in zip(degrees, wins)]
+    blocks = [gaussian_block(30 + i, A.shape[0], w) for i, w in enumerate(widths)]
+    out = sm.apply_poly_slices(A, blocks, co, smin, smax, precision="fp64", ctx=ctx)
+    for i, Q in enumerate(blocks):
+        Yo = clenshaw_c.apply_poly_block(A, Q, co[i], smin, smax)
+        err = np.max(np.abs(out[i] - Yo)) / np.max(np.abs(Yo))
+        assert err <= 1e-12, (i, err)
+
+
+def test_c2_polfed_mixed_and_fp64_full_size(ctx, c2):
+    """The benchmark's C2 solve: ~500 pairs at E=0 in an interval sized by
+    KPM, learned parameters, FP64 and mixed precision."""
+    import sparse_matrix_algorithms_amd as sm
+    from sparse_matrix_algorithms_amd import kpm
+    from sparse_matrix_algorithms_amd.learned import MLP, choose_parameters
+    A, smin, smax = c2
+    n = A.shape[0]
+    anorm = max(abs(smin), abs(smax))
+    dm = sm.DeviceMatrix(A, ctx)
+    try:
+        mu, _, _ = kpm.kpm_moments(dm, 256, 16, 7, smin, smax, ctx=ctx)
+        emin, emax = kpm.interval_for_count(mu, n, 0.96 * 500, smin, smax, 0.0)
+        est = kpm.count(mu, n, emin, emax, smin, smax)
+        model = MLP.load(ctx=ctx)
+        common = dict(s=64, smin=smin, smax=smax, sigma=0.0, emin=emin, emax=emax,
+                      tol=1e-10 * anorm, ctx=ctx, seed=1)
+        sets = {}
+        for prec in ("fp64", "fp32"):
+            K, mm, _ = choose_parameters(model, mu[:64], n, 500, emin, emax, smin, smax, 64,
+                                         precision=prec)
+            lam, X, res = sm.polfed(dm, 500, K, mmax=mm, precision=prec, **common)
+            ok = (lam >= emin) & (lam <= emax) & (res <= 1e-10 * anorm)
+            assert ok.all(), (prec, int(ok.sum()), len(lam))
+            assert len(lam) >= 0.9 * est, (prec, len(lam), est)
+            # residuals and orthonormality recomputed on the host in FP64
+            R = A @ X - X * lam[None, :]
+            rn = np.linalg.norm(R, axis=0)
+            assert rn.max() <= 1.01e-10 * anorm, (prec, rn.max() / anorm)
+            assert np.allclose(rn, res, rtol=1e-3, atol=1e-14 * anorm), prec
+            G = X.T @ X
+            assert np.max(np.abs(G - np.eye(len(lam)))) <= 1e-8, prec
+            sets[prec] = np.sort(lam)
+        a, b = sets["fp64"], sets["fp32"]
+        assert len(a) == len(b)
+        assert np.max(np.abs(a - b)) <= 1e-10 * anorm
+    finally:
+        dm.free()
