@@ -6,6 +6,12 @@ copy kernel of known size (bytes read = bytes written = ``--copy-mib`` MiB)
 so that FETCH_SIZE / WRITE_SIZE counters can be scaled to bytes.
 
     python -m sparse_matrix_algorithms_amd.profile_filter --L 48 --b 64 --prec fp32 --degree 50
+
+With ``--slices N`` it runs the packed multi-slice filter instead: N slices of
+widths 8, 64, 16, 32, ... (the batched-slices benchmark's widths, cycled),
+all of degree ``--degree``, packed and launched as the batched driver does
+(HIP graphs), and prints the library's own per-launch time for comparison
+with the profiler's kernel average.
 """
 from __future__ import annotations
 
@@ -15,6 +21,10 @@ import sys
 import time
 
 import numpy as np
+
+
+# block widths of the batched-slices benchmark (nev 50..400 at L=64)
+SLICE_WIDTHS = [8, 64, 16, 32, 8, 32, 16, 64, 8, 32, 16, 32, 16, 64, 32, 32]
 
 
 def main():
@@ -27,6 +37,7 @@ def main():
     ap.add_argument("--degree", type=int, default=50)
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--slices", type=int, default=0)
     args = ap.parse_args()
 
     import sparse_matrix_algorithms_amd as sm
@@ -38,6 +49,18 @@ def main():
     dm = sm.DeviceMatrix(A, ctx)
     smin, smax = dm.gershgorin_bounds()
     co = sm.build_coeffs(args.degree, smin, smax, 0.0, -0.04, 0.04, "step")
+    if args.slices:
+        widths = [SLICE_WIDTHS[i % len(SLICE_WIDTHS)] for i in range(args.slices)]
+        blocks = [gaussian_block(1 + i, n, w) for i, w in enumerate(widths)]
+        _, st = sm.apply_poly_slices(dm, blocks, [co] * len(widths), smin, smax,
+                                     precision=args.prec, return_stats=True, ctx=ctx,
+                                     replays=args.reps)
+        print(f"packed filter: {args.slices} slices, widths {widths}, degree {args.degree}, "
+              f"n={n}, {args.prec}: panel width {st['panel_width']:.0f}, "
+              f"{st['launches']:.0f} launches per run, "
+              f"{st['ms_per_run'] / st['launches'] * 1e3:.2f} us per launch, "
+              f"{st['bytes_per_run'] / st['ms_per_run'] / 1e6:.0f} GB/s algorithmic", flush=True)
+        return 0
     Q = gaussian_block(1, n, args.b)
     for r in range(args.reps):
         t = time.perf_counter()
