@@ -255,8 +255,10 @@ def main():
             detail["fp64_" + src], set2 = summary(l2, r2, s2, d2, KL64, mmaxL64)
             detail["fp64_" + src]["same_set_as_fp64_plain"] = same(set2, set64)
             log(f"[bench] fp64 {src}: {d2:.3f}s {s2['line']}")
-            l3, _, r3, s3, d3 = solve("fp32", K, mmax)
-            detail["mixed_plain"], set3 = summary(l3, r3, s3, d3, K, mmax)
+            Km, mmaxm = kpm.plain_parameters(n, args.nev, emin, emax, smin, smax, args.s,
+                                             args.kappa, args.mmax_factor, precision="fp32")
+            l3, _, r3, s3, d3 = solve("fp32", Km, mmaxm)
+            detail["mixed_plain"], set3 = summary(l3, r3, s3, d3, Km, mmaxm)
             detail["mixed_plain"]["same_set_as_fp64_plain"] = same(set3, set64)
             log(f"[bench] mixed plain: {d3:.3f}s {s3['line']}")
 
@@ -391,7 +393,7 @@ def bench_slices(args):
                                            precision=prec)
         else:
             K, mm = kpm.plain_parameters(n, nev, a, b, smin, smax, s, args.kappa,
-                                         args.mmax_factor)
+                                         args.mmax_factor, precision=prec)
         slices.append(dict(emin=a, emax=b, sigma=c, nev=nev, s=s, degree=K, mmax=mm))
         log(f"[bench] slice {i}: centre {c:+.3f} [{a:.5f},{b:.5f}] nev={nev} s={s} K={K} "
             f"mmax={mm} ({src})")
@@ -426,7 +428,8 @@ def bench_slices(args):
             for i in redo:
                 d = slices[i]
                 K, mm = kpm.plain_parameters(n, d["nev"], d["emin"], d["emax"], smin, smax,
-                                             d["s"], args.kappa, args.mmax_factor)
+                                             d["s"], args.kappa, args.mmax_factor,
+                                             precision=prec)
                 plain.append(dict(d, degree=K, mmax=mm))
                 log(f"[bench] slice {i} under-converged: again with the plain rule K={K} "
                     f"mmax={mm}")
@@ -525,7 +528,8 @@ def bench_xxz(args):
     e0 = float(A.diagonal().sum() / n)
     mu, _, _ = kpm.kpm_moments(dm, args.kpm_moments, args.kpm_probes, 7, smin, smax, ctx=ctx)
     emin, emax = kpm.interval_for_count(mu, n, args.fill * nev, smin, smax, e0)
-    K, mm = kpm.plain_parameters(n, nev, emin, emax, smin, smax, s, args.kappa, args.mmax_factor)
+    K, mm = kpm.plain_parameters(n, nev, emin, emax, smin, smax, s, args.kappa, args.mmax_factor,
+                                 precision=prec)
     log(f"[bench] xxz L={L} n={n} nnz={dm.nnz} centre {e0:.4f} [{emin:.5f},{emax:.5f}] "
         f"K={K} mmax={mm}")
     kw = dict(s=s, smin=smin, smax=smax, sigma=e0, emin=emin, emax=emax, tol=1e-10 * anorm,
@@ -555,7 +559,9 @@ def bench_xxz(args):
         "dtype": "fp64" if prec == "fp64" else "fp32+fp64",
         "data": "synthetic (seeded random fields)",
         "config": {"workload": f"POLFED random-field XXZ L={L} Sz=0 n={n}, J=1 Delta=1 h=5, "
-                               f"~{nev} pairs at tr(H)/n, s={s}, {prec}", "L": L, "n": n,
+                               f"~{nev} pairs at tr(H)/n, s={s}, {prec}, plain parameters"
+                               + (" (degree x0.7 for the mixed path)" if prec == "fp32" else ""),
+                   "L": L, "n": n,
                    "nev": nev, "block": s, "degree": K, "mmax": mm,
                    "parallelism": "single GPU"},
         "roofline": roof, "cpu_baseline": cpu,

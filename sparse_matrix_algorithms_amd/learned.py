@@ -40,7 +40,8 @@ def margin_bucket(blocks: float) -> int:
 KAPPA_RANGE = (3.0, 10.0)
 FACTOR_RANGE = (1.6, 7.0)
 WEIGHTS = Path(__file__).resolve().parent / "data" / "mlp_weights.json"
-MIXED_DEGREE_SCALE = 0.8   # learned degree factor for the mixed path
+MIXED_DEGREE_SCALE = 0.7   # learned degree factor for the mixed path
+MIXED_MMAX_SCALE = 1.08    # learned basis-size factor for the mixed path (rounded up)
 
 
 def _fp(a):
@@ -151,11 +152,14 @@ def choose_parameters(model: Optional[MLP], mu: np.ndarray, n: int, nev: int, em
     """(degree, mmax, source) from the network plus its calibrated safety
     margins, or from the plain rule when the predicted basis undercounts the
     stochastic estimate (or no model).  For the mixed path (precision
-    "fp32") the learned degree is scaled by MIXED_DEGREE_SCALE: its FP64
-    refinement is one more filtered subspace step on the candidates, so the
-    FP32 basis converges with a lower degree (held-out check in
-    data/heldout_eval_mixed.json)."""
-    K0, m0 = plain_parameters(n, nev, emin, emax, smin, smax, s)
+    "fp32") the learned degree is scaled by MIXED_DEGREE_SCALE and the basis
+    size by MIXED_MMAX_SCALE: its FP64 refinement is one more filtered
+    subspace step on the candidates, so the FP32 basis converges with a lower
+    degree, and the refinement's cost falls with the degree while the basis
+    grows only by a block or two (held-out check in
+    data/heldout_eval_mixed.json).  The plain-rule answers (no model, or
+    fallback) take the plain rule's own mixed-path scale."""
+    K0, m0 = plain_parameters(n, nev, emin, emax, smin, smax, s, precision=precision)
     if model is None:
         return K0, m0, "plain"
     meta = getattr(model, "meta", {})
@@ -174,4 +178,5 @@ def choose_parameters(model: Optional[MLP], mu: np.ndarray, n: int, nev: int, em
         return K0, m0, "fallback"
     if precision == "fp32":
         K = max(1, math.ceil(MIXED_DEGREE_SCALE * K))
+        mmax = max(1, math.ceil(MIXED_MMAX_SCALE * mmax - 1e-9))
     return K, mmax, "learned"

@@ -157,12 +157,15 @@ def train(path: Path, epochs: int = 3000, lr: float = 3e-3, out: Path = None):
 
 
 def evaluate(path: Path, weights: Path = None, margin_kappa=None, margin_factor=None,
-             precision: str = "fp64", degree_scale: float = 1.0, refine_scale=None):
+             precision: str = "fp64", degree_scale: float = 1.0, refine_scale=None,
+             rule: str = "learned", mmax_scale: float = 1.0):
     """Held-out instances (256..319): learned parameters (with the stochastic
     count fallback) versus the plain rule solved on the GPU in FP64.  The
     learned run uses `precision` and the learned degree times `degree_scale`
     (the mixed path's FP64 refinement is one more filtered subspace step, so
-    its basis can use a lower degree).
+    its basis can use a lower degree).  With rule="plain" the candidate run
+    takes the plain rule's parameters instead (degree times `degree_scale`).
+    `mmax_scale` multiplies the candidate's basis size (rounded up).
     A run succeeds when it returns at least 0.9 of the KPM count of the
     window at relative residual ≤ 1e-10 and, where the plain run succeeded,
     the same eigenvalue set as the plain run."""
@@ -177,10 +180,15 @@ def evaluate(path: Path, weights: Path = None, margin_kappa=None, margin_factor=
         dm = sm.DeviceMatrix(A, ctx)
         anorm = max(abs(r["smin"]), abs(r["smax"]))
         mu = np.array(r["mu"])
-        K, mmax, src = choose_parameters(model, mu, r["n"], r["nev"], r["emin"], r["emax"],
-                                         r["smin"], r["smax"], S, margin_kappa, margin_factor)
+        if rule == "plain":
+            K, mmax, src = r["plain"]["degree"], r["plain"]["mmax"], "plain"
+        else:
+            K, mmax, src = choose_parameters(model, mu, r["n"], r["nev"], r["emin"], r["emax"],
+                                             r["smin"], r["smax"], S, margin_kappa,
+                                             margin_factor)
         if src != "fallback":  # the scale is applied here, on the FP64 rule's degree
             K = max(1, math.ceil(degree_scale * K))
+            mmax = max(1, math.ceil(mmax_scale * mmax - 1e-9))
         kr = None if refine_scale is None else max(1, math.ceil(refine_scale * K))
         lam, t, st = _solve(sm, dm, r["nev"], K, mmax, r["emin"], r["emax"], r["smin"],
                             r["smax"], anorm, ctx, precision, kr)
@@ -200,7 +208,8 @@ def evaluate(path: Path, weights: Path = None, margin_kappa=None, margin_factor=
               f"(K={K}, mmax={mmax}) plain {t0:.3f}s (K={K0}, mmax={m0})", flush=True)
         dm.free()
     both = [x for x in rows if x["plain_ok"]]
-    summ = dict(instances=len(rows), precision=precision, degree_scale=degree_scale,
+    summ = dict(instances=len(rows), rule=rule, precision=precision, degree_scale=degree_scale,
+                mmax_scale=mmax_scale,
                 refine_scale=refine_scale,
                 success_learned=sum(x["ok"] for x in rows),
                 success_plain=sum(x["plain_ok"] for x in rows),
@@ -232,6 +241,10 @@ def main():
                     help="eval: factor on the learned degree")
     ap.add_argument("--refine-scale", type=float, default=None,
                     help="eval: refinement degree as a factor of the basis degree")
+    ap.add_argument("--mmax-scale", type=float, default=1.0,
+                    help="eval: factor on the candidate's basis size")
+    ap.add_argument("--rule", choices=["learned", "plain"], default="learned",
+                    help="eval: parameters of the candidate runs")
     ap.add_argument("--redo-unconverged", type=Path, default=None,
                     help="regenerate the instances of this log whose plain run did not converge")
     a = ap.parse_args()
@@ -246,7 +259,7 @@ def main():
         train(a.data, a.epochs, a.lr, a.out)
     else:
         summ, rows = evaluate(a.data, precision=a.precision, degree_scale=a.degree_scale,
-                              refine_scale=a.refine_scale)
+                              refine_scale=a.refine_scale, rule=a.rule, mmax_scale=a.mmax_scale)
         if a.out:
             Path(a.out).write_text(json.dumps(dict(summary=summ, rows=rows), indent=1))
 

@@ -80,3 +80,28 @@ def test_committed_dataset_and_weights():
     assert len(w["params"]) == learned.NIN * 128 + 128 + 128 * 2 + 2
     ev = json.loads((learned.WEIGHTS.parent / "heldout_eval.json").read_text())["summary"]
     assert ev["instances"] == 64 and ev["success_learned"] >= ev["success_plain"]
+
+
+def test_mixed_path_scales():
+    """The mixed path (precision "fp32") lowers the degree and widens the
+    basis of the learned answer, and lowers the plain rule's degree."""
+    import math
+    from sparse_matrix_algorithms_amd import kpm
+    mu = _mu()
+    n, nev, s, smin, smax = 10000, 120, 32, -10.0, 10.0
+    emin, emax = -0.1, 0.1
+    K64, m64, _ = learned.choose_parameters(_Fixed(5.0, 3.0), mu, n, nev, emin, emax, smin,
+                                            smax, s)
+    K32, m32, src = learned.choose_parameters(_Fixed(5.0, 3.0), mu, n, nev, emin, emax, smin,
+                                              smax, s, precision="fp32")
+    assert src == "learned"
+    assert K32 == math.ceil(learned.MIXED_DEGREE_SCALE * K64) < K64
+    assert m32 == math.ceil(learned.MIXED_MMAX_SCALE * m64) > m64
+    Kp, mp = plain_parameters(n, nev, emin, emax, smin, smax, s)
+    Kq, mq = plain_parameters(n, nev, emin, emax, smin, smax, s, precision="fp32")
+    assert Kq == math.ceil(kpm.MIXED_PLAIN_DEGREE_SCALE * Kp) and mq == mp
+    # no model or fallback: the plain rule with its own mixed-path scale
+    assert learned.choose_parameters(None, mu, n, nev, emin, emax, smin, smax, s,
+                                     precision="fp32")[:2] == (Kq, mq)
+    with pytest.raises(ValueError):
+        plain_parameters(n, nev, emin, emax, smin, smax, s, precision="bf16")
