@@ -38,7 +38,7 @@ extern "C" {
 #define SMALGO_WINDOW_DELTA 1
 #define SMALGO_WINDOW_SMOOTH 2
 
-/* precision of the filter / basis (north-star mixed mode; FP64 = reference) */
+/* precision of the filter / basis (FP32 basis + FP64 refinement; FP64 = reference) */
 #define SMALGO_PREC_FP64 0
 #define SMALGO_PREC_FP32_REFINE64 1
 
@@ -208,7 +208,7 @@ typedef struct smalgo_polfed_params {
   uint64_t seed;         /* default 1 (start block when Q1 == NULL) */
   int chunk;             /* default 64 (kept for signature parity; the device lifts all at once) */
   int show_progress;     /* default 0: print the progress line (polfed.jl:991-1000) */
-  /* mixed precision (north star; no reference counterpart) */
+  /* mixed precision (no reference counterpart) */
   int precision;         /* SMALGO_PREC_* (default FP64) */
   int refine_guard;      /* guard vectors in FP64 refinement (default -1: nev, at most the
                             Ritz vectors left after the candidates) */
