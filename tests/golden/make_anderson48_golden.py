@@ -1,7 +1,7 @@
 """Eigenvalues of the L=48 Anderson workload near E=0, by a method independent
 of the filter solvers under test.
 
-    python tests/golden/make_anderson_c2_golden.py      # ~15 min, ~6 GB
+    python tests/golden/make_anderson48_golden.py      # ~15 min, ~6 GB
 
 Matrix: 3D Anderson L=48 (n=110,592), W=16.5, disorder seed 48, the matrix the
 POLFED benchmark solves.

@@ -2,7 +2,7 @@
 
     python tests/golden/make_anderson_golden.py
 
-C1 workload: 3D Anderson L=20 (n=8000), W=16.5, disorder seed 20, all
+Chebyshev workload: 3D Anderson L=20 (n=8000), W=16.5, disorder seed 20, all
 eigenvalues in [-0.2, 0.2] from numpy.linalg.eigvalsh (LAPACK syevd) of the
 dense matrix.  Also records the Gershgorin bounds and a checksum of the
 onsite energies, so a change in the generator is caught.

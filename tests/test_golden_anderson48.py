@@ -1,5 +1,5 @@
 """Self-consistency of the L=48 Anderson eigenvalue fixture near E=0
-(``golden/make_anderson_c2_golden.py``: shift-invert Lanczos, with the count
+(``golden/make_anderson48_golden.py``: shift-invert Lanczos, with the count
 fixed by Sylvester inertia), and a check that the generator still builds the
 matrix the fixture was made from."""
 import json

@@ -1,5 +1,5 @@
-"""Parity at the benchmark sizes: C2 (Anderson L=48, n=110,592), C3 (XXZ L=20,
-n=184,756) and C4 (Anderson L=64, n=262,144).
+"""Parity at the benchmark sizes: 3D Anderson L=48 (n=110,592), the XXZ chain
+L=20 (n=184,756) and 3D Anderson L=64 (n=262,144).
 
 The filter (``apply_poly_block!``, polfed.jl:464-545) is compared with the
 oracle's OpenMP FP64 Clenshaw (``oracle/clenshaw.c``, pinned to the NumPy
@@ -8,7 +8,7 @@ cover the whole-width launch, the cache-aware column panels (blocks whose
 three streamed n×b arrays exceed the cache budget) and the packed
 multi-slice filter at full size.
 
-The full C2 solve (``polfed``, polfed.jl:1128-1158) is checked through
+The full L=48 Anderson solve (``polfed``, polfed.jl:1128-1158) is checked through
 properties that hold at any size:
 - FP64 residuals and orthonormality of the returned vectors, recomputed on
   the host;
@@ -17,7 +17,7 @@ properties that hold at any size:
   eigenvalue count.
 It is also checked against the committed shift-invert Lanczos eigenvalues of
 the same matrix, whose count near E=0 is fixed exactly by Sylvester inertia
-(``golden/make_anderson_c2_golden.py``): the same count and values in the
+(``golden/make_anderson48_golden.py``): the same count and values in the
 solved interval.
 """
 import json
@@ -32,22 +32,22 @@ from sparse_matrix_algorithms_amd.lattices import anderson3d, gaussian_block, xx
 
 pytestmark = pytest.mark.gpu
 
-GOLDEN_C2 = Path(__file__).resolve().parent / "golden" / "anderson_L48_W16.5_seed48_centre.json"
+GOLDEN_L48 = Path(__file__).resolve().parent / "golden" / "anderson_L48_W16.5_seed48_centre.json"
 
 
 @pytest.fixture(scope="module")
-def c2():
+def anderson48():
     A = anderson3d(48, 16.5, 48)
     smin, smax = P.gershgorin_bounds(A)
     return A, smin, smax
 
 
 @pytest.mark.parametrize("b,K", [(64, 200), (128, 100)])
-def test_c2_filter_fp64_vs_c_oracle(ctx, c2, b, K):
+def test_anderson48_filter_fp64_vs_c_oracle(ctx, anderson48, b, K):
     """b=64: one launch over the whole width; b=128 FP64 (340 MB of streamed
     blocks): split into cache-aware panels."""
     import sparse_matrix_algorithms_amd as sm
-    A, smin, smax = c2
+    A, smin, smax = anderson48
     Q = gaussian_block(21, A.shape[0], b)
     co = P.build_coeffs(K, smin, smax, 0.0, -0.04, 0.04, "step")
     Y = sm.apply_poly_block(A, Q, co, smin, smax, ctx=ctx)
@@ -56,10 +56,10 @@ def test_c2_filter_fp64_vs_c_oracle(ctx, c2, b, K):
     assert err <= 1e-12, err
 
 
-def test_c2_filter_fp32_vs_c_oracle(ctx, c2):
-    """FP32 filter at C2: FP32 rounding only (1e-4 of the block norm)."""
+def test_anderson48_filter_fp32_vs_c_oracle(ctx, anderson48):
+    """FP32 filter at L=48: FP32 rounding only (1e-4 of the block norm)."""
     import sparse_matrix_algorithms_amd as sm
-    A, smin, smax = c2
+    A, smin, smax = anderson48
     Q = gaussian_block(22, A.shape[0], 64)
     co = P.build_coeffs(200, smin, smax, 0.0, -0.04, 0.04, "step")
     Y = sm.apply_poly_block(A, Q, co, smin, smax, precision="fp32", ctx=ctx)
@@ -68,7 +68,7 @@ def test_c2_filter_fp32_vs_c_oracle(ctx, c2):
     assert err <= 1e-4, err
 
 
-def test_c3_filter_fp64_vs_c_oracle(ctx):
+def test_xxz20_filter_fp64_vs_c_oracle(ctx):
     """XXZ rows hold 1..11 nonzeros; b=128 FP64 runs in panels at this n."""
     import sparse_matrix_algorithms_amd as sm
     A = xxz_sz0(20, 5.0, 20)
@@ -81,7 +81,7 @@ def test_c3_filter_fp64_vs_c_oracle(ctx):
     assert err <= 1e-12, err
 
 
-def test_c4_packed_slices_vs_c_oracle(ctx):
+def test_anderson64_packed_slices_vs_c_oracle(ctx):
     """Packed filter over four slices of different width and degree at
     n=262,144 (one of them degree 0), each against the oracle."""
     import sparse_matrix_algorithms_amd as sm
@@ -99,13 +99,13 @@ def test_c4_packed_slices_vs_c_oracle(ctx):
         assert err <= 1e-12, (i, err)
 
 
-def test_c2_polfed_mixed_and_fp64_full_size(ctx, c2):
-    """The benchmark's C2 solve: ~500 pairs at E=0 in an interval sized by
+def test_anderson48_polfed_mixed_and_fp64_full_size(ctx, anderson48):
+    """The benchmark's L=48 Anderson solve: ~500 pairs at E=0 in an interval sized by
     KPM, learned parameters, FP64 and mixed precision."""
     import sparse_matrix_algorithms_amd as sm
     from sparse_matrix_algorithms_amd import kpm
     from sparse_matrix_algorithms_amd.learned import MLP, choose_parameters
-    A, smin, smax = c2
+    A, smin, smax = anderson48
     n = A.shape[0]
     anorm = max(abs(smin), abs(smax))
     dm = sm.DeviceMatrix(A, ctx)
@@ -136,7 +136,7 @@ def test_c2_polfed_mixed_and_fp64_full_size(ctx, c2):
         assert len(a) == len(b)
         assert np.max(np.abs(a - b)) <= 1e-10 * anorm
         # every eigenvalue of the interval, from shift-invert Lanczos + inertia
-        g = json.loads(GOLDEN_C2.read_text())
+        g = json.loads(GOLDEN_L48.read_text())
         assert g["n"] == n and np.isclose(g["diag_sum"], A.diagonal().sum(), rtol=1e-12)
         assert g["s_lo"] < emin and emax < g["s_hi"], (emin, emax, g["s_lo"], g["s_hi"])
         ev = np.array(g["eigenvalues"])

@@ -1,4 +1,4 @@
-"""Known answers on the GPU: the C1 workload against its committed dense-eigh
+"""Known answers on the GPU: the Chebyshev benchmark cube (3D Anderson L=20) against its committed dense-eigh
 fixture, and the reference's published Lap1D POLFED run
 (polfed.jl:1279-1288: n=10,000, nev=1000, s=64, degree=32, mmax=79 ->
 inside=1000, maxres 3.1e-14)."""
@@ -23,7 +23,7 @@ def _c1():
     return A, g
 
 
-def test_c1_chebyshev_fp64_matches_dense_eigh(ctx):
+def test_anderson20_chebyshev_fp64_matches_dense_eigh(ctx):
     import sparse_matrix_algorithms_amd as sm
     A, g = _c1()
     ref = np.array(g["eigenvalues"])
@@ -38,7 +38,7 @@ def test_c1_chebyshev_fp64_matches_dense_eigh(ctx):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_c1_polfed_matches_dense_eigh(ctx, precision):
+def test_anderson20_polfed_matches_dense_eigh(ctx, precision):
     import sparse_matrix_algorithms_amd as sm
     A, g = _c1()
     ref = np.array(g["eigenvalues"])

@@ -56,7 +56,7 @@ def test_argument_errors():
 
 
 def test_anderson_dense_golden_c1_scaled():
-    """C1-style run (Chebyshev, FP64) on a small Anderson cube vs dense eigh."""
+    """Chebyshev FP64 run as on the L=20 benchmark cube, on a small Anderson cube vs dense eigh."""
     A = anderson3d(8, 16.5, 20)
     ev = np.linalg.eigh(A.toarray())[0]
     emin, emax = -0.5, 0.5
