@@ -1,0 +1,41 @@
+"""bench.py's host-side rules, on the CPU: the valid-pair count, the
+under-converged slice test, and the one-GPU guard."""
+import importlib.util
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench", ROOT / "bench.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_valid_pairs_and_under_converged():
+    b = _bench()
+    anorm = 10.0
+    lam = np.array([-0.2, 0.0, 0.1, 0.3, 0.5])
+    res = np.array([1e-12, 5e-10, 1e-3, 1e-12, 1e-12])
+    n, ok = b.valid_pairs(lam, res, -0.25, 0.35, anorm)
+    assert n == 3 and ok.tolist() == [True, True, False, True, False]
+    # a spurious Ritz value (residual ~1e-3·|A|) alone is not under-convergence
+    assert not b.under_converged(lam, res, -0.25, 0.35, anorm)
+    # a real pair just above the 1e-10·|A| bound is
+    res2 = res.copy()
+    res2[1] = 3e-9
+    assert b.under_converged(lam, res2, -0.25, 0.35, anorm)
+    # outside the interval it does not count
+    assert not b.under_converged(lam, res2, 0.05, 0.35, anorm)
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
