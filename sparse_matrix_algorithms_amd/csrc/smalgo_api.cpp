@@ -409,17 +409,13 @@ int smalgo_copy_bench(smalgo_ctx* ctx, int64_t bytes, int reps, double* seconds)
     DBuf<float> a(nb / 4), b(nb / 4);
     fill_block<float>(ctx->stream, 1, (int)(nb / 4), a.p, (int)(nb / 4), 1.0f);
     stream_copy(ctx->stream, a.p, b.p, nb);
-    hipEvent_t e0, e1;
-    SMALGO_HIP(hipEventCreate(&e0));
-    SMALGO_HIP(hipEventCreate(&e1));
+    Event e0, e1;
     SMALGO_HIP(hipEventRecord(e0, ctx->stream));
     for (int r = 0; r < reps; ++r) stream_copy(ctx->stream, a.p, b.p, nb);
     SMALGO_HIP(hipEventRecord(e1, ctx->stream));
     SMALGO_HIP(hipEventSynchronize(e1));
     float ms = 0;
     SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     *seconds = ms * 1e-3 / reps;
   });
 }

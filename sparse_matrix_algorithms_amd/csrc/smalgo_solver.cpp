@@ -65,11 +65,12 @@ void Dense::syevd(int n, double* A, int lda, double* w, double* work_e, bool upp
 // phase timer
 // ---------------------------------------------------------------------------
 size_t PhaseTimer::begin(const std::string& name) {
-  hipEvent_t a, b;
+  hipEvent_t a = nullptr, b = nullptr;
   SMALGO_HIP(hipEventCreate(&a));
+  ev.push_back({name, {a, nullptr}});  // owned by the timer from here on
   SMALGO_HIP(hipEventCreate(&b));
+  ev.back().second.second = b;
   SMALGO_HIP(hipEventRecord(a, st));
-  ev.push_back({name, {a, b}});
   return ev.size() - 1;
 }
 void PhaseTimer::end(size_t h) { SMALGO_HIP(hipEventRecord(ev[h].second.second, st)); }
@@ -85,8 +86,8 @@ std::map<std::string, double> PhaseTimer::resolve() {
 }
 PhaseTimer::~PhaseTimer() {
   for (auto& e : ev) {
-    (void)hipEventDestroy(e.second.first);
-    (void)hipEventDestroy(e.second.second);
+    if (e.second.first) (void)hipEventDestroy(e.second.first);
+    if (e.second.second) (void)hipEventDestroy(e.second.second);
   }
 }
 
@@ -283,9 +284,7 @@ static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ld
     }
   }
   if (cand.empty()) return def;
-  hipEvent_t e0, e1;
-  SMALGO_HIP(hipEventCreate(&e0));
-  SMALGO_HIP(hipEventCreate(&e1));
+  Event e0, e1;
   FilterTile best = def;
   float best_ms = 1e30f;
   for (const FilterTile& t : cand) {
@@ -305,8 +304,6 @@ static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ld
       best = t;
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   cx.tuned[key] = best;
   return best;
 }
@@ -1118,9 +1115,7 @@ struct SliceFilter {
           if (w % v == 0 && l * v <= std::max(w, 4) * 2 && l * v * (int)sizeof(T) <= 2048)
             for (int rt : {1, 2, 4}) cand.push_back(FilterTile{v, l, rt});
     }
-    hipEvent_t e0, e1;
-    SMALGO_HIP(hipEventCreate(&e0));
-    SMALGO_HIP(hipEventCreate(&e1));
+    Event e0, e1;
     FilterTile best = cand.front();
     float best_ms = 1e30f;
     for (const FilterTile& tl : cand) {
@@ -1140,8 +1135,6 @@ struct SliceFilter {
         best = tl;
       }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     cx.tuned[key] = best;
     cx.tuned_ms[key] = best_ms;
     *ms_out = best_ms;
@@ -2107,9 +2100,7 @@ static void apply_poly_slices_t(Ctx& cx, const Op& op, const std::vector<int>& w
   sf.plan(widths, co, smin, smax, pw_force);
   const int ld = round_up(std::max(wtot, 1), 4);
   DBuf<T> blk((size_t)n * ld);
-  hipEvent_t e0, e1;
-  SMALGO_HIP(hipEventCreate(&e0));
-  SMALGO_HIP(hipEventCreate(&e1));
+  Event e0, e1;
   double ms_tot = 0;
   for (int r = 0; r < replays; ++r) {
     // the host round trip between replays is ordinary eager work
@@ -2133,8 +2124,6 @@ static void apply_poly_slices_t(Ctx& cx, const Op& op, const std::vector<int>& w
     SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
     ms_tot += ms;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (stats) {
     stats[0] = sf.max_width();
     stats[1] = (double)sf.pk.pan.size();

@@ -40,6 +40,18 @@ struct DBuf {
   ~DBuf() { release(); }
 };
 
+// A timing event that is destroyed on every exit path of its scope.
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() { SMALGO_HIP(hipEventCreate(&e)); }
+  Event(const Event&) = delete;
+  Event& operator=(const Event&) = delete;
+  ~Event() {
+    if (e) (void)hipEventDestroy(e);
+  }
+  operator hipEvent_t() const { return e; }
+};
+
 // Pinned host staging for host-callback operators: two grow-only pinned
 // blocks, a second stream for the host->device leg and per-chunk events.
 struct Staging {
