@@ -637,7 +637,8 @@ __global__ __launch_bounds__(256) void colred_kernel(int n, int b, int rows_per_
 }
 
 // one workgroup per group of 16 columns: 16 threads per column stride over the
-// partial sums, then a fixed-order tree in LDS (deterministic)
+// partial sums with four independent accumulators (so four loads are in
+// flight per thread), then a fixed-order tree in LDS (deterministic)
 template <int MODE>
 __global__ __launch_bounds__(256) void colred_finish(int b, int nparts,
                                                      const double* __restrict__ part, double* out) {
@@ -645,8 +646,16 @@ __global__ __launch_bounds__(256) void colred_finish(int b, int nparts,
   const int cl = threadIdx.x & 15, pl = threadIdx.x >> 4;
   const int c = blockIdx.x * 16 + cl;
   double s = 0.0;
-  if (c < b)
-    for (int p = pl; p < nparts; p += 16) s += part[(size_t)p * b + c];
+  if (c < b) {
+    double s4[4] = {0.0, 0.0, 0.0, 0.0};
+    int p = pl;
+    for (; p + 48 < nparts; p += 64) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s4[k] += part[(size_t)(p + 16 * k) * b + c];
+    }
+    for (int k = 0; p < nparts; p += 16, ++k) s4[k & 3] += part[(size_t)p * b + c];
+    s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+  }
   red[pl][cl] = s;
   __syncthreads();
   for (int h = 8; h > 0; h >>= 1) {
