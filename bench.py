@@ -41,12 +41,15 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("anderson", "slices", "xxz", "cheb"),
+    ap.add_argument("--workload", choices=("anderson", "slices", "xxz", "cheb", "lap1d"),
                     default="anderson",
                     help="anderson: one interval of the L=48 Anderson model (the headline); "
                          "slices: 16 intervals of the L=64 Anderson model in one batched run; "
                          "xxz: ~1000 pairs at the centre of the L=20 random-field XXZ chain; "
-                         "cheb: Chebyshev subspace iteration on L=20 Anderson, [-0.2, 0.2]")
+                         "cheb: Chebyshev subspace iteration on L=20 Anderson, [-0.2, 0.2]; "
+                         "lap1d: the reference's own published POLFED runs on the 1D Laplacian")
+    ap.add_argument("--lap1d-run", default="n150000",
+                    help="lap1d: which published run (a label of LAP1D_RUNS, or 'all')")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default=None,
                     help="slices/xxz: arithmetic of the basis build (fp32 = FP32 filter + FP64 "
                          "refinement); default fp64")
@@ -135,6 +138,8 @@ def main():
         return bench_xxz(args)
     if args.workload == "cheb":
         return bench_cheb(args)
+    if args.workload == "lap1d":
+        return bench_lap1d(args)
     args.L = args.L or 48
     args.seed = args.L if args.seed is None else args.seed
     args.s = args.s or 64
@@ -569,6 +574,107 @@ def bench_xxz(args):
                        pairs=int(len(lam)), t_filter=st["t_filter"], t_orth=st["t_orth"],
                        t_extract=st["t_extract"], t_refine=st.get("t_refine", 0.0),
                        max_rel_res=float(res.max() / anorm) if len(res) else None),
+    }
+    print(json.dumps(out), flush=True)
+
+
+# The reference's published POLFED runs on tridiag(-1, 2, -1) (polfed.jl:1279-1407; Apple
+# M3 Max, matrix-free, FP64): label -> (n, nev, degree, mmax, published seconds of the
+# build_full_basis! progress bar, published inside count, published maxres).  Every run
+# used s=64, tol=1e-6, overextract=2, extra_keep=8 and the delta window (the call at
+# polfed.jl:1260 passes window_type=:delta whatever the log line says).
+LAP1D_RUNS = {
+    "n10000": (10_000, 1000, 32, 79, 19.0, 1000, 3.1e-14),
+    "n60000": (60_000, 1000, 128, 47, 36.0, 999, 2.2e-12),
+    "n150000": (150_000, 1000, 400, 47, 133.0, 998, 7.5e-14),
+    "n300000": (300_000, 1000, 750, 47, 416.0, 998, 9.4e-14),
+    "n350000": (350_000, 1600, 800, 75, 891.0, 1599, 2.1e-11),
+    "n450000_K728": (450_000, 1000, 728, 79, 1418.0, 998, 1.2e-13),
+    "n450000_K1250": (450_000, 1000, 1250, 47, 937.0, 1007, 1.9e-2),
+    "n650000": (650_000, 1000, 980, 79, 2289.0, 1000, 1.2e-13),
+    "n1048576_K3500": (1_048_576, 1000, 3500, 32, 3491.0, 1000, 2.1e-13),
+    "n1048576_K2500": (1_048_576, 1000, 2500, 47, 7993.0, 999, 2.6e-11),
+    "n1048576_K1600": (1_048_576, 1000, 1600, 94, 14104.0, 999, 2.5e-11),
+}
+
+
+def bench_lap1d(args):
+    """The reference's own published POLFED runs on the 1D Laplacian, same
+    parameters, on the GPU: CSR form of the same tridiagonal matrix, FP64,
+    window by count at the centre of the spectrum (polfed.jl:1223-1233).
+    Reported per run: the basis-build time (the phases the published progress
+    bar covers) and the whole solve, the inside count and maxres against the
+    published ones, and the eigenvalue error against the closed form.
+    value = eigenpairs/s of the whole solve (pairs at relative residual
+    <= 1e-10), for the selected run or the first of `all`."""
+    import sparse_matrix_algorithms_amd as sm
+    from sparse_matrix_algorithms_amd.lattices import lap1d, lap1d_exact_eigs
+    labels = list(LAP1D_RUNS) if args.lap1d_run == "all" else [args.lap1d_run]
+    for lb in labels:
+        if lb not in LAP1D_RUNS:
+            raise SystemExit(f"unknown --lap1d-run {lb}; one of {', '.join(LAP1D_RUNS)} or all")
+    ctx = sm.default_context()
+    runs, first, cpu = [], None, None
+    for lb in labels:
+        n, nev, K, mmax, pub_s, pub_inside, pub_maxres = LAP1D_RUNS[lb]
+        A = lap1d(n)
+        lex = lap1d_exact_eigs(n)
+        mid = min(max(round(n / 2), 1), n)                  # lap1d_target_window
+        i1 = min(max(mid - nev // 2, 1), n)
+        i2 = min(max(i1 + nev - 1, 1), n)
+        i1 = min(max(i2 - nev + 1, 1), n)
+        emin, emax = lex[i1 - 1], lex[i2 - 1]
+        sigma = 0.5 * (emin + emax)
+        dm = sm.DeviceMatrix(A, ctx)
+        kw = dict(s=64, smin=0.0, smax=4.0, sigma=sigma, emin=emin, emax=emax, tol=1e-6,
+                  window_type="delta", overextract=2, extra_keep=8, ctx=ctx,
+                  return_stats=True, want_X=False, precision=args.precision or "fp64")
+        sm.polfed(dm, nev, 8, mmax=2, **kw)                 # warm: tiles, buffers
+        t = time.perf_counter()
+        lam, _, res, st = sm.polfed(dm, nev, K, mmax=mmax, **kw)
+        dt = time.perf_counter() - t
+        t_basis = st["t_filter"] + st["t_orth"]
+        ok = res <= 1e-10 * 4.0
+        d = np.abs(np.sort(lam)[:, None] - lex[None, i1 - 1:i2]).min(axis=1) if len(lam) else []
+        inside = int(st["line"].split("inside=")[1].split()[0])
+        row = dict(run=lb, n=n, nev=nev, degree=K, mmax=mmax, seconds=round(dt, 3),
+                   basis_seconds=round(t_basis, 3), published_basis_seconds=pub_s,
+                   speedup_basis_vs_published=round(pub_s / t_basis, 1),
+                   inside=inside, published_inside=pub_inside, pairs=int(len(lam)),
+                   valid=int(ok.sum()), maxres=float(res.max()) if len(res) else None,
+                   published_maxres=pub_maxres,
+                   max_eig_error=float(np.max(d)) if len(lam) else None,
+                   filter_gbs=round(st["spmm_bytes"] / st["t_filter"] / 1e9, 1),
+                   line=st["line"])
+        runs.append(row)
+        log(f"[bench] lap1d {lb}: solve {dt:.2f}s, basis {t_basis:.2f}s (published {pub_s:.0f}s, "
+            f"{pub_s / t_basis:.0f}x), inside {inside} (published {pub_inside}), "
+            f"valid {row['valid']}, max |dlambda| {row['max_eig_error']:.1e}")
+        if first is None:
+            first = (row, st, A, emin, emax)
+        dm.free()
+    row, st, A, emin, emax = first
+    if not args.skip_cpu:
+        cpu = cpu_baseline(A, 64, 0.0, 4.0, emin, emax, st["filter_steps"], row["valid"],
+                           args.cpu_seconds, "of the solve")
+    prec = args.precision or "fp64"
+    out = {
+        "metric": "eigenpairs/s at rel. resid <=1e-10, POLFED on the reference's published "
+                  "1D Laplacian runs",
+        "value": round(row["valid"] / row["seconds"], 3), "unit": "eigenpairs/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(row["seconds"] * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp64" if prec == "fp64" else "fp32+fp64",
+        "data": "1D Laplacian tridiag(-1,2,-1), window by count at the centre",
+        "config": {"workload": f"POLFED Lap1D published run {row['run']}: n={row['n']}, "
+                               f"nev={row['nev']}, s=64, degree {row['degree']}, "
+                               f"mmax {row['mmax']}, delta window, tol 1e-6, {prec}",
+                   "n": row["n"], "nev": row["nev"], "degree": row["degree"],
+                   "mmax": row["mmax"], "parallelism": "single GPU"},
+        "roofline": filter_roofline(st, prec, False), "cpu_baseline": cpu,
+        "detail": dict(runs=runs, note="published seconds cover build_full_basis! only "
+                                       "(polfed.jl:1277); basis_seconds is the same phases "
+                                       "(filter + orthogonalisation) here"),
     }
     print(json.dumps(out), flush=True)
 
