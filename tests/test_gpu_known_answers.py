@@ -1,7 +1,8 @@
 """Known answers on the GPU: the Chebyshev benchmark cube (3D Anderson L=20) against its committed dense-eigh
-fixture, and the reference's published Lap1D POLFED run
-(polfed.jl:1279-1288: n=10,000, nev=1000, s=64, degree=32, mmax=79 ->
-inside=1000, maxres 3.1e-14)."""
+fixture, and the reference's published Lap1D POLFED runs
+(polfed.jl:1279-1310: n=10,000, 60,000 and 150,000 at nev=1000, s=64 ->
+inside=1000, 999 and 998) and Chebyshev run (chebyshev_subspace_iteration.jl:
+1123-1136)."""
 import json
 import math
 from pathlib import Path
@@ -81,6 +82,30 @@ def test_published_lap1d_polfed_run(ctx, window):
     tol = 1e-12 if window == "delta" else 1e-10
     assert d.max() <= tol
     assert res.max() <= tol
+
+
+@pytest.mark.parametrize("n,degree,mmax,published_inside",
+                         [(60_000, 128, 47, 999), (150_000, 400, 47, 998)])
+def test_published_lap1d_polfed_runs_at_scale(ctx, n, degree, mmax, published_inside):
+    """The reference's published POLFED runs at n=60,000 and n=150,000
+    (polfed.jl:1290-1310: nev=1000, s=64, b=3.0 -> mmax=47, delta window,
+    tol 1e-6 -> inside=999 and 998, maxres 2.2e-12 and 7.5e-14).  The start
+    block is the library's, so the inside count may differ by the edge pairs;
+    every returned pair must sit on the closed-form spectrum."""
+    import sparse_matrix_algorithms_amd as sm
+    nev = 1000
+    lex, i1, i2, emin, emax, sigma = target_window(n, nev)
+    lam, _, res, st = sm.polfed(lap1d(n), nev, degree, s=64, smin=0.0, smax=4.0, sigma=sigma,
+                                emin=emin, emax=emax, tol=1e-6, window_type="delta", mmax=mmax,
+                                overextract=2, extra_keep=8, return_stats=True, ctx=ctx,
+                                want_X=False)
+    inside = int(st["line"].split("inside=")[1].split()[0])
+    assert st["block_steps"] == mmax
+    assert abs(inside - published_inside) <= 2, st["line"]
+    assert len(lam) >= nev - 3
+    d = np.abs(np.sort(lam)[:, None] - lex[None, i1 - 1:i2]).min(axis=1)
+    assert d.max() <= 1e-12
+    assert res.max() <= 1e-12
 
 
 def test_published_lap1d_chebyshev_delta_run(ctx):
