@@ -596,6 +596,44 @@ LAP1D_RUNS = {
     "n1048576_K2500": (1_048_576, 1000, 2500, 47, 7993.0, 999, 2.6e-11),
     "n1048576_K1600": (1_048_576, 1000, 1600, 94, 14104.0, 999, 2.5e-11),
 }
+# The published Chebyshev run (chebyshev_subspace_iteration.jl:1123-1136): n, target
+# count, m0, degree, iterations, published seconds of the whole call, matched targets.
+LAP1D_CHEB_RUN = ("cheb_n50000", 50_000, 1000, 2000, 128, 7, 117.08, 1000)
+
+
+def bench_lap1d_cheb(sm, ctx):
+    """The published Chebyshev run: window by count with two eigenvalues of
+    padding on each side (chebyshev_subspace_iteration.jl:980-992), delta
+    window, tol 1e-8, res_gate 1e-6, exactly 7 iterations (the sparse form
+    runs maxiter iterations)."""
+    from sparse_matrix_algorithms_amd.lattices import lap1d, lap1d_exact_eigs
+    lb, n, nt, m0, K, iters, pub_s, pub_matched = LAP1D_CHEB_RUN
+    lex = lap1d_exact_eigs(n)
+    ic = min(max(int(np.round(0.5 * n)), 1), n)
+    i1 = max(1, ic - nt // 2)
+    i2 = min(n, i1 + nt - 1)
+    i1 = max(1, i2 - nt + 1)
+    emin, emax = lex[max(1, i1 - 2) - 1], lex[min(n, i2 + 2) - 1]
+    sigma = 0.5 * (lex[i1 - 1] + lex[i2 - 1])
+    dm = sm.DeviceMatrix(lap1d(n), ctx)
+    kw = dict(smin=0.0, smax=4.0, sigma=sigma, m0=m0, degree=K, tol=1e-8, res_gate=1e-6,
+              window_type="delta", two_hit=False, return_stats=True, ctx=ctx)
+    sm.chebyshev_poly_real_symm_sparse(dm, emin, emax, **dict(kw, maxiter=1, degree=8))
+    t = time.perf_counter()
+    lam, _, res, st = sm.chebyshev_poly_real_symm_sparse(dm, emin, emax, maxiter=iters, **kw)
+    dt = time.perf_counter() - t
+    target = lex[i1 - 1:i2]
+    d = np.abs(target[:, None] - np.sort(lam)[None, :]).min(axis=1) if len(lam) else target
+    matched = int(np.sum(d <= 1e-8))
+    dm.free()
+    row = dict(run=lb, n=n, nev=nt, m0=m0, degree=K, iterations=iters, seconds=round(dt, 3),
+               published_seconds=pub_s, speedup_vs_published=round(pub_s / dt, 1),
+               matched=matched, published_matched=pub_matched, pairs=int(len(lam)),
+               maxres=float(res.max()) if len(res) else None,
+               max_eig_error=float(d.max()) if len(lam) else None)
+    log(f"[bench] lap1d {lb}: solve {dt:.2f}s (published {pub_s:.0f}s, {pub_s / dt:.0f}x), "
+        f"matched {matched}/{nt} (published {pub_matched}), pairs {len(lam)}")
+    return row
 
 
 def bench_lap1d(args):
@@ -610,9 +648,14 @@ def bench_lap1d(args):
     import sparse_matrix_algorithms_amd as sm
     from sparse_matrix_algorithms_amd.lattices import lap1d, lap1d_exact_eigs
     labels = list(LAP1D_RUNS) if args.lap1d_run == "all" else [args.lap1d_run]
+    with_cheb = args.lap1d_run in ("all", LAP1D_CHEB_RUN[0])
+    labels = [lb for lb in labels if lb != LAP1D_CHEB_RUN[0]]
     for lb in labels:
         if lb not in LAP1D_RUNS:
-            raise SystemExit(f"unknown --lap1d-run {lb}; one of {', '.join(LAP1D_RUNS)} or all")
+            raise SystemExit(f"unknown --lap1d-run {lb}; one of {', '.join(LAP1D_RUNS)}, "
+                             f"{LAP1D_CHEB_RUN[0]} or all")
+    if not labels:
+        raise SystemExit("--lap1d-run needs at least one POLFED run (or all)")
     ctx = sm.default_context()
     runs, first, cpu = [], None, None
     for lb in labels:
@@ -653,6 +696,7 @@ def bench_lap1d(args):
         if first is None:
             first = (row, st, A, emin, emax)
         dm.free()
+    cheb = bench_lap1d_cheb(sm, ctx) if with_cheb else None
     row, st, A, emin, emax = first
     if not args.skip_cpu:
         cpu = cpu_baseline(A, 64, 0.0, 4.0, emin, emax, st["filter_steps"], row["valid"],
@@ -672,7 +716,8 @@ def bench_lap1d(args):
                    "n": row["n"], "nev": row["nev"], "degree": row["degree"],
                    "mmax": row["mmax"], "parallelism": "single GPU"},
         "roofline": filter_roofline(st, prec, False), "cpu_baseline": cpu,
-        "detail": dict(runs=runs, note="published seconds cover build_full_basis! only "
+        "detail": dict(runs=runs, chebyshev=cheb,
+                       note="published seconds cover build_full_basis! only "
                                        "(polfed.jl:1277); basis_seconds is the same phases "
                                        "(filter + orthogonalisation) here"),
     }
