@@ -34,6 +34,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}  # MI355X dense matrix peaks
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 
@@ -205,8 +206,9 @@ def main():
     nvalid, ok = valid_pairs(lam, res, emin, emax, anorm)
     value = nvalid / t_step
 
-    def summary(lam_, res_, st_, dt_, K_, mm_):
+    def summary(lam_, res_, st_, dt_, K_, mm_, prec_):
         nv, ok_ = valid_pairs(lam_, res_, emin, emax, anorm)
+        tf = st_["gemm_flops"] / max(st_["t_orth"], 1e-9) / 1e12
         return dict(seconds=round(dt_, 4), degree=K_, mmax=mm_, pairs=int(len(lam_)), valid=nv,
                     eigenpairs_per_s=round(nv / dt_, 3),
                     max_rel_res=float(res_.max() / anorm) if len(res_) else None,
@@ -217,10 +219,13 @@ def main():
                     t_refine_host=st_.get("t_refine_host", 0.0),
                     locked=st_.get("locked", 0),
                     filter_gbs=round(st_["spmm_bytes"] / st_["t_filter"] / 1e9, 1),
-                    orth_gemm_tflops=round(st_["gemm_flops"] / max(st_["t_orth"], 1e-9) / 1e12,
-                                           3)), np.sort(lam_[ok_])
+                    orth_gemm_tflops=round(tf, 3),
+                    # MFMA use of the block-orthogonalisation phase (Gram and update GEMMs
+                    # plus the small FP64 Cholesky steps), against the dense matrix peak of
+                    # its GEMM precision
+                    orth_mfma_frac=round(tf / MFMA_PEAK_TFLOPS[prec_], 4)), np.sort(lam_[ok_])
 
-    head, head_set = summary(lam, res, st, t_step, KL, mmaxL)
+    head, head_set = summary(lam, res, st, t_step, KL, mmaxL, head_prec)
     detail = dict(n=n, nnz=dm.nnz, interval=[emin, emax], s=args.s, parameter_source=src,
                   kpm_seconds=round(t_k, 3), upload_seconds=round(t_up, 3),
                   headline=dict(precision=head_prec, **head))
@@ -249,7 +254,7 @@ def main():
 
     if not args.skip_fp64:
         lam64, X64, res64, st64, dt64 = solve("fp64", K, mmax)
-        detail["fp64_plain"], set64 = summary(lam64, res64, st64, dt64, K, mmax)
+        detail["fp64_plain"], set64 = summary(lam64, res64, st64, dt64, K, mmax, "fp64")
         detail["same_eigenvalue_set_as_fp64_plain"] = same(head_set, set64)
         detail["speedup_vs_fp64_plain"] = round(value / max(detail["fp64_plain"]
                                                             ["eigenpairs_per_s"], 1e-12), 3)
@@ -257,13 +262,13 @@ def main():
         st64_steps = st64["filter_steps"]
         if not args.skip_variants:
             l2, _, r2, s2, d2 = solve("fp64", KL64, mmaxL64)
-            detail["fp64_" + src], set2 = summary(l2, r2, s2, d2, KL64, mmaxL64)
+            detail["fp64_" + src], set2 = summary(l2, r2, s2, d2, KL64, mmaxL64, "fp64")
             detail["fp64_" + src]["same_set_as_fp64_plain"] = same(set2, set64)
             log(f"[bench] fp64 {src}: {d2:.3f}s {s2['line']}")
             Km, mmaxm = kpm.plain_parameters(n, args.nev, emin, emax, smin, smax, args.s,
                                              args.kappa, args.mmax_factor, precision="fp32")
             l3, _, r3, s3, d3 = solve("fp32", Km, mmaxm)
-            detail["mixed_plain"], set3 = summary(l3, r3, s3, d3, Km, mmaxm)
+            detail["mixed_plain"], set3 = summary(l3, r3, s3, d3, Km, mmaxm, "fp32")
             detail["mixed_plain"]["same_set_as_fp64_plain"] = same(set3, set64)
             log(f"[bench] mixed plain: {d3:.3f}s {s3['line']}")
 
