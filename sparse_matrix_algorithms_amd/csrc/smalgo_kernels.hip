@@ -449,14 +449,35 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(int n, int m, int b, int k
       }
 }
 
+// Sum of the split-K partials.  A workgroup covers 256/zg consecutive elements
+// with zg threads per element; thread group g sums partials g, g+zg, ... with
+// two independent accumulators, then group sums combine in LDS in a fixed
+// order (deterministic).  Small Gram matrices have few elements and many
+// partials, so the partials, not the elements, carry the parallelism there.
 template <typename TM, typename TC>
-__global__ void gemm_tn_reduce(int m, int b, int nsplit, const TM* __restrict__ part, double alpha,
-                               double beta, TC* C, int ldc) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)m * b) return;
-  const int i = (int)(idx / b), j = (int)(idx % b);
+__global__ __launch_bounds__(256) void gemm_tn_reduce(int m, int b, int nsplit, int zg,
+                                                      const TM* __restrict__ part, double alpha,
+                                                      double beta, TC* C, int ldc) {
+  __shared__ double red[256];
+  const int epb = 256 / zg;
+  const int le = threadIdx.x % epb, g = threadIdx.x / epb;
+  const size_t tot = (size_t)m * b;
+  const size_t idx = (size_t)blockIdx.x * epb + le;
+  double s0 = 0.0, s1 = 0.0;
+  if (idx < tot) {
+    int z = g;
+    for (; z + zg < nsplit; z += 2 * zg) {
+      s0 += (double)part[(size_t)z * tot + idx];
+      s1 += (double)part[(size_t)(z + zg) * tot + idx];
+    }
+    if (z < nsplit) s0 += (double)part[(size_t)z * tot + idx];
+  }
+  red[threadIdx.x] = s0 + s1;
+  __syncthreads();
+  if (g != 0 || idx >= tot) return;
   double s = 0.0;
-  for (int z = 0; z < nsplit; ++z) s += (double)part[(size_t)z * m * b + idx];
+  for (int q = 0; q < zg; ++q) s += red[q * epb + le];
+  const int i = (int)(idx / b), j = (int)(idx % b);
   double v = alpha * s;
   if (beta != 0.0) v += beta * (double)C[(size_t)i * ldc + j];
   C[(size_t)i * ldc + j] = (TC)v;
@@ -493,8 +514,11 @@ void gemm_tn(hipStream_t st, int n, int m, int b, double alpha, const TA* A, int
     SMALGO_HIP(hipMemsetAsync(part, 0, sizeof(TM) * (size_t)m * b, st));
   }
   const size_t tot = (size_t)m * b;
-  hipLaunchKernelGGL((gemm_tn_reduce<TM, TC>), dim3((tot + 255) / 256), dim3(256), 0, st, m, b,
-                     n > 0 ? nsplit : 1, part, alpha, beta, C, ldc);
+  const int ns = n > 0 ? nsplit : 1;
+  const int zg = ns >= 16 ? 16 : (ns >= 4 ? 4 : 1);  // threads per element
+  const size_t epb = 256 / zg;
+  hipLaunchKernelGGL((gemm_tn_reduce<TM, TC>), dim3((unsigned)((tot + epb - 1) / epb)), dim3(256), 0,
+                     st, m, b, ns, zg, part, alpha, beta, C, ldc);
   SMALGO_HIP(hipGetLastError());
 }
 
