@@ -22,7 +22,9 @@ using Random
 using SparseArrays
 
 export polfed, polfed_slices, chebyshev_poly_real_symm_sparse, MatrixFreeRealSymOp, gershgorin_bounds
-export save_csr, load_csr
+export save_csr, load_csr, subspace_rr
+export jackson_kernel, cheb_delta_coeffs, cheb_window_coeffs, cheb_smooth_window_coeffs
+export apply_poly_block!, POLFEDWorkspace, Chebyshev_workspace
 
 const libsmalgo = get(ENV, "SMALGO_LIB", joinpath(@__DIR__, "..", "..", "..", "libsmalgo.so"))
 
@@ -148,6 +150,131 @@ function take_result(h::Ptr{Cvoid})
     ccall((:smalgo_result_free, libsmalgo), Cvoid, (Ptr{Cvoid},), h)
     check(rc)
     return λ, X, res
+end
+
+# ---- Chebyshev coefficients (polfed.jl:275-414), host code of the library ----------
+argcheck(rc) = rc == 0 ? nothing : throw(ArgumentError("invalid coefficient arguments"))
+
+"`jackson_kernel(K)` (polfed.jl:275-284): K+1 damping factors, `[1.0]` for K <= 0."
+function jackson_kernel(K::Int)
+    g = zeros(max(K, 0) + 1)
+    argcheck(ccall((:smalgo_jackson_kernel, libsmalgo), Cint, (Cint, Ptr{Cdouble}), K, g))
+    return g
+end
+
+"`cheb_delta_coeffs(σ̂, K; jackson)` (polfed.jl:299-313)."
+function cheb_delta_coeffs(σ̂::Float64, K::Int; jackson::Bool=true)
+    c = zeros(K + 1)
+    argcheck(ccall((:smalgo_cheb_delta_coeffs, libsmalgo), Cint, (Cdouble, Cint, Cint, Ptr{Cdouble}),
+                   σ̂, K, jackson, c))
+    return c
+end
+
+"`cheb_window_coeffs(a, b, K; jackson)` (polfed.jl:329-347)."
+function cheb_window_coeffs(a::Float64, b::Float64, K::Int; jackson::Bool=true)
+    c = zeros(K + 1)
+    argcheck(ccall((:smalgo_cheb_window_coeffs, libsmalgo), Cint,
+                   (Cdouble, Cdouble, Cint, Cint, Ptr{Cdouble}), a, b, K, jackson, c))
+    return c
+end
+
+"`cheb_smooth_window_coeffs(a, b, K; jackson, nsamp, δ)` (polfed.jl:364-388)."
+function cheb_smooth_window_coeffs(a::Float64, b::Float64, K::Int; jackson::Bool=true,
+                                   nsamp::Int=4096, δ::Float64=0.02)
+    c = zeros(K + 1)
+    argcheck(ccall((:smalgo_cheb_smooth_window_coeffs, libsmalgo), Cint,
+                   (Cdouble, Cdouble, Cint, Cint, Cint, Cdouble, Ptr{Cdouble}),
+                   a, b, K, jackson, nsamp, δ, c))
+    return c
+end
+
+"`_build_coeffs(degree, smin, smax, sigma, emin, emax; window_type, jackson)` (polfed.jl:404-414)."
+function _build_coeffs(degree, smin, smax, sigma, emin, emax; window_type::Symbol=:step,
+                       jackson::Bool=true)
+    haskey(WINDOWS, window_type) || error("Unknown window type: $window_type")
+    c = zeros(Int(degree) + 1)
+    argcheck(ccall((:smalgo_build_coeffs, libsmalgo), Cint,
+                   (Cint, Cdouble, Cdouble, Cdouble, Cdouble, Cdouble, Cint, Cint, Ptr{Cdouble}),
+                   degree, smin, smax, sigma, emin, emax, WINDOWS[window_type], jackson, c))
+    return c
+end
+
+# ---- block filter apply_poly_block! (polfed.jl:464-545) ----------------------------
+"""
+    POLFEDWorkspace(n, s), Chebyshev_workspace(n, b)
+
+Accepted for signature parity with `apply_poly_block!`; the recurrence's
+temporaries live in device memory owned by the library.
+"""
+struct POLFEDWorkspace
+    n::Int
+    s::Int
+end
+const Chebyshev_workspace = POLFEDWorkspace
+
+"""
+    apply_poly_block!(Y, A, Q, coeffs, smin, smax, work) -> Y
+
+`Y = p(Â) Q` with Â = (A − cI)/d and p = Σ coeffs[k+1] T_k (polfed.jl:464-545),
+on the GPU in FP64.  `A` is a `SparseMatrixCSC` (uploaded on each call; keep a
+`DeviceMatrix` to reuse it) or a `MatrixFreeRealSymOp`.
+"""
+function apply_poly_block!(Y::AbstractMatrix{Float64}, A::Union{SparseMatrixCSC{Float64,Int},DeviceMatrix},
+                           Q::AbstractMatrix{Float64}, coeffs::AbstractVector{Float64},
+                           smin::Float64, smax::Float64, work=nothing)
+    dev = A isa DeviceMatrix ? A : DeviceMatrix(A)
+    n, b = size(Q)
+    (size(Y) == (n, b) && n == dev.n) || throw(DimensionMismatch("Y, Q and A sizes differ"))
+    Qc = Matrix{Float64}(Q); Yc = Matrix{Float64}(undef, n, b); co = Vector{Float64}(coeffs)
+    GC.@preserve Qc Yc co begin
+        check(ccall((:smalgo_apply_poly_block, libsmalgo), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cdouble}, Int64, Ptr{Cdouble}, Cint,
+                     Cdouble, Cdouble, Cint, Ptr{Cdouble}, Int64),
+                    context().h, dev.h, b, Qc, n, co, length(co) - 1, smin, smax, 0, Yc, n))
+    end
+    copyto!(Y, Yc)
+    return Y
+end
+
+function apply_poly_block!(Y::AbstractMatrix{Float64}, A::MatrixFreeRealSymOp,
+                           Q::AbstractMatrix{Float64}, coeffs::AbstractVector{Float64},
+                           smin::Float64, smax::Float64, work=nothing)
+    n, b = size(Q)
+    (size(Y) == (n, b) && n == A.n) || throw(DimensionMismatch("Y, Q and A sizes differ"))
+    Qc = Matrix{Float64}(Q); Yc = Matrix{Float64}(undef, n, b); co = Vector{Float64}(coeffs)
+    GC.@preserve A Qc Yc co begin
+        check(ccall((:smalgo_apply_poly_block_mf, libsmalgo), Cint,
+                    (Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cdouble}, Int64,
+                     Ptr{Cdouble}, Cint, Cdouble, Cdouble, Ptr{Cdouble}, Int64),
+                    context().h, n, tramp(), pointer_from_objref(A), b, Qc, n, co,
+                    length(co) - 1, smin, smax, Yc, n))
+    end
+    copyto!(Y, Yc)
+    return Y
+end
+
+# ---- Rayleigh–Ritz tail for FEAST / Zolotarev (feast.jl:540-553, zoloterov.jl:1698-1706)
+"""
+    subspace_rr(A, Y) -> (λ, X, res, Q)
+
+thin_orth(Y) → A·Q → Qᵀ(AQ) → eigen → X = Q·Xq → residuals on the GPU, for all
+columns of Y in ascending λ; `Q` is thin_orth(Y).  The shifted solves that form
+Y stay on the host.
+"""
+function subspace_rr(A::Union{SparseMatrixCSC{Float64,Int},DeviceMatrix}, Y::AbstractMatrix{Float64})
+    dev = A isa DeviceMatrix ? A : DeviceMatrix(A)
+    n, k = size(Y)
+    n == dev.n || throw(DimensionMismatch("Y and A sizes differ"))
+    Yc = Matrix{Float64}(Y); Qc = zeros(n, k)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve Yc Qc begin
+        check(ccall((:smalgo_subspace_rr, libsmalgo), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Ptr{Cdouble},
+                     Int64, Ptr{Cdouble}, Int64, Ptr{Ptr{Cvoid}}),
+                    context().h, dev.h, C_NULL, C_NULL, n, k, Yc, n, Qc, n, h))
+    end
+    λ, X, res = take_result(h[])
+    return λ, X, res, Qc
 end
 
 # ---- POLFED -------------------------------------------------------------------

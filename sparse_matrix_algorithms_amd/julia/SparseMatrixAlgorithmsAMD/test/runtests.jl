@@ -34,3 +34,36 @@ end
     @test all(r -> r < 1e-6, res)
     @test length(λ) >= 38
 end
+
+@testset "coefficients and apply_poly_block! against the spectral form" begin
+    @test jackson_kernel(0) == [1.0]
+    g = jackson_kernel(40)
+    @test length(g) == 41 && g[1] ≈ 1.0
+    # a Chebyshev series evaluated at cos θ from its coefficients
+    cheb(c, x) = sum(c[k+1] * cos(k * acos(clamp(x, -1, 1))) for k in 0:length(c)-1)
+    c = cheb_window_coeffs(-0.3, 0.2, 200)
+    @test cheb(c, 0.0) ≈ 1.0 atol = 0.02
+    @test abs(cheb(c, 0.8)) < 0.02
+    n = 300
+    A = lap1d(n)
+    λall = lap1d_eigs(n)
+    V = [sqrt(2 / (n + 1)) * sin(i * k * pi / (n + 1)) for i in 1:n, k in 1:n]   # eigenvectors
+    co = SparseMatrixAlgorithmsAMD._build_coeffs(60, 0.0, 4.0, 2.0, 1.8, 2.2; window_type=:step)
+    Q = randn(n, 5)
+    Y = similar(Q)
+    apply_poly_block!(Y, A, Q, co, 0.0, 4.0, POLFEDWorkspace(n, 5))
+    p = [cheb(co, (λ - 2.0) / 2.0) for λ in λall]
+    @test maximum(abs.(Y .- V * (p .* (V' * Q)))) < 1e-12 * maximum(abs.(Y))
+end
+
+@testset "subspace_rr on an exact invariant subspace" begin
+    n = 400
+    A = lap1d(n)
+    λall = lap1d_eigs(n)
+    V = [sqrt(2 / (n + 1)) * sin(i * k * pi / (n + 1)) for i in 1:n, k in 191:210]
+    Y = V * randn(20, 20)             # same span, mixed columns
+    λ, X, res, Q = subspace_rr(A, Y)
+    @test maximum(abs.(λ .- λall[191:210])) < 1e-12
+    @test maximum(res) < 1e-12
+    @test maximum(abs.(Q' * Q - I)) < 1e-12
+end
