@@ -643,20 +643,24 @@ __global__ __launch_bounds__(256) void colred_kernel(int n, int b, int rows_per_
   if (rs >= nrt) return;
   const int r0 = blockIdx.x * rows_per_blk;
   const int r1 = min(n, r0 + rows_per_blk);
+  auto term = [&](int i, int c, double l) {
+    const double x = (double)X[(size_t)i * ldx + c];
+    const double y = (double)Y[(size_t)i * ldy + c];
+    if (MODE == 0) return x * y;
+    const double d = y - l * x;
+    return d * d;
+  };
   for (int c = tid % nct; c < b; c += nct) {
-    double s = 0.0;
     const double l = MODE == 1 ? lam[c] : 0.0;
-    for (int i = r0 + rs; i < r1; i += nrt) {
-      const double x = (double)X[(size_t)i * ldx + c];
-      const double y = (double)Y[(size_t)i * ldy + c];
-      if (MODE == 0) {
-        s += x * y;
-      } else {
-        const double d = y - l * x;
-        s += d * d;
-      }
+    // two independent accumulators keep two rows' loads in flight
+    double s0 = 0.0, s1 = 0.0;
+    int i = r0 + rs;
+    for (; i + nrt < r1; i += 2 * nrt) {
+      s0 += term(i, c, l);
+      s1 += term(i + nrt, c, l);
     }
-    part[((size_t)blockIdx.x * nrt + rs) * b + c] = s;
+    if (i < r1) s0 += term(i, c, l);
+    part[((size_t)blockIdx.x * nrt + rs) * b + c] = s0 + s1;
   }
 }
 
