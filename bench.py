@@ -80,6 +80,9 @@ def parse():
     ap.add_argument("--refine-guard", type=int, default=None)
     ap.add_argument("--refine-degree", type=int, default=None)
     ap.add_argument("--refine-maxiter", type=int, default=3)
+    ap.add_argument("--refine-guard-degree", type=int, default=None,
+                    help="guard vectors' filter degree in the first refinement iteration "
+                         "(default: a tenth of the refinement degree)")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -185,7 +188,8 @@ def main():
         extra = {}
         if precision == "fp32":
             extra = dict(refine_guard=args.refine_guard, refine_degree=args.refine_degree,
-                         refine_maxiter=args.refine_maxiter)
+                         refine_maxiter=args.refine_maxiter,
+                         refine_guard_degree=args.refine_guard_degree)
         t = time.perf_counter()
         lam, X, res, st = sm.polfed(dm, args.nev, degree, mmax=mm, precision=precision, **common,
                                     **extra)
@@ -409,7 +413,8 @@ def bench_slices(args):
             f"mmax={mm} ({src})")
     kw = dict(smin=smin, smax=smax, tol=1e-10 * anorm, precision=prec, ctx=ctx, want_X=False,
               refine_guard=args.refine_guard, refine_degree=args.refine_degree,
-              refine_maxiter=args.refine_maxiter)
+              refine_maxiter=args.refine_maxiter,
+              refine_guard_degree=args.refine_guard_degree)
 
     def batch(sls):
         t = time.perf_counter()
@@ -545,7 +550,8 @@ def bench_xxz(args):
     kw = dict(s=s, smin=smin, smax=smax, sigma=e0, emin=emin, emax=emax, tol=1e-10 * anorm,
               precision=prec, ctx=ctx, return_stats=True, want_X=False, mmax=mm,
               refine_guard=args.refine_guard, refine_degree=args.refine_degree,
-              refine_maxiter=args.refine_maxiter)
+              refine_maxiter=args.refine_maxiter,
+              refine_guard_degree=args.refine_guard_degree)
     for w in range(args.warmup):
         sm.polfed(dm, nev, K, **kw)
     times = []

@@ -216,6 +216,9 @@ typedef struct smalgo_polfed_params {
   int refine_maxiter;    /* FP64 refinement iterations with locking (default 3) */
   double refine_tol;     /* lock a pair when ‖r‖ < refine_tol·‖A‖; default -1: when ‖r‖ < tol,
                             the caller's absolute tolerance of the return rule */
+  int refine_guard_degree; /* degree of the guard vectors' filter in the first refinement
+                              iteration (default -1: a tenth of the refinement degree; the
+                              guards only widen the Rayleigh–Ritz subspace) */
 } smalgo_polfed_params;
 
 void smalgo_polfed_default_params(smalgo_polfed_params* p);

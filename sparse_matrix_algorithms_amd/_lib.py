@@ -33,7 +33,7 @@ class PolfedParams(C.Structure):
         ("reorth_passes", C.c_int), ("extra_keep", C.c_int), ("seed", C.c_uint64),
         ("chunk", C.c_int), ("show_progress", C.c_int), ("precision", C.c_int),
         ("refine_guard", C.c_int), ("refine_degree", C.c_int), ("refine_maxiter", C.c_int),
-        ("refine_tol", C.c_double),
+        ("refine_tol", C.c_double), ("refine_guard_degree", C.c_int),
     ]
 
 

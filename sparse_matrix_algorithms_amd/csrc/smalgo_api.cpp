@@ -502,6 +502,7 @@ void smalgo_polfed_default_params(smalgo_polfed_params* p) {
   p->refine_degree = -1;
   p->refine_maxiter = 3;
   p->refine_tol = -1.0;
+  p->refine_guard_degree = -1;
 }
 
 void smalgo_cheb_default_params(smalgo_cheb_params* p) {

@@ -1442,17 +1442,20 @@ struct ResidualFilterWork {
 // already small for Ritz vectors, so Y keeps FP64 accuracy at FP32 bytes.
 static void residual_filter(Ctx& cx, const Op& op, int k, const double* Z, int ldz,
                             const std::vector<double>& th, const std::vector<double>& co,
-                            double smin, double smax, double* Y, int ldy, ResidualFilterWork& w) {
+                            double smin, double smax, double* Y, int ldy, ResidualFilterWork& w,
+                            int kmain = -1, const std::vector<double>* cog = nullptr) {
   const int n = op.n;
   if (k == 0) return;
   const auto h0 = std::chrono::steady_clock::now();
   const double c = 0.5 * (smin + smax), d = 0.5 * (smax - smin);
-  const int K = (int)co.size() - 1;
-  std::vector<std::vector<double>> q(k);
+  // columns [kmain, k) (guard vectors) may take their own, lower-degree window
+  if (kmain < 0 || kmain > k || cog == nullptr) kmain = k;
+  const int K = std::max((int)co.size() - 1, cog && kmain < k ? (int)cog->size() - 1 : 0);
+  std::vector<std::vector<double>> q(kmain), qg(k - kmain);
   std::vector<double> pa(k), mth(k);
   for (int i = 0; i < k; ++i) {
     const double a = std::min(1.0, std::max(-1.0, (th[i] - c) / d));
-    pa[i] = cheb_divide(co, a, q[i]);
+    pa[i] = i < kmain ? cheb_divide(co, a, q[i]) : cheb_divide(*cog, a, qg[i - kmain]);
     mth[i] = -th[i] / d;
   }
   const int ld = round_up(k, 4);
@@ -1472,14 +1475,26 @@ static void residual_filter(Ctx& cx, const Op& op, int k, const double* Z, int l
     launch_clenshaw_step_cols<double>(cx.stream, *op.A, k, Z, ldz, w.cqd.p, Z, ldz, 1.0 / d, 0.0,
                                       nullptr, 0, 0.0, w.Rd.p, ld, t);
     copy_block<double, float>(cx.stream, n, k, w.Rd.p, ld, w.Rf.p, ld);
-    // W = q_i(Â) r̂_i: one slice of width k, one coefficient vector per column
+    // W = q_i(Â) r̂_i: one coefficient vector per column; the guard columns,
+    // when they take a lower degree, form a second slice with its own panels
     SliceFilter<float> sf(cx, op);
     std::vector<SliceCoeffs> cs{SliceCoeffs{nullptr, &q}};
-    sf.plan({k}, cs, smin, smax, cache_panel_width<float>(op));
+    std::vector<int> widths{kmain};
+    if (kmain < k) {
+      cs.push_back(SliceCoeffs{nullptr, &qg});
+      widths.push_back(k - kmain);
+    }
+    if (kmain == 0) {
+      cs.erase(cs.begin());
+      widths.erase(widths.begin());
+    }
+    sf.plan(widths, cs, smin, smax, cache_panel_width<float>(op));
     w.host_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
-    sf.load(0, w.Rf.p, ld);
+    for (int i = 0, off = 0; i < (int)widths.size(); off += widths[i], ++i)
+      sf.load(i, w.Rf.p + off, ld);
     sf.run(false);
-    sf.store(0, w.Wf.p, ld);
+    for (int i = 0, off = 0; i < (int)widths.size(); off += widths[i], ++i)
+      sf.store(i, w.Wf.p + off, ld);
     copy_block<float, double>(cx.stream, n, k, w.Wf.p, ld, Y, ldy);
   } else {
     fill_block<double>(cx.stream, n, k, Y, ldy, 0.0);
@@ -1617,6 +1632,17 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
         const std::vector<double> cr =
             build_coeffs(Kr, smin, smax, p.sigma, p.emin, p.emax, p.window_type, p.jackson != 0);
         const double tol_abs = p.refine_tol >= 0 ? p.refine_tol * anorm : p.tol;
+        // The guard vectors only widen the Rayleigh–Ritz subspace by directions just
+        // outside the window, which the FP32 basis already carries; in the first
+        // iteration they take a window of refine_guard_degree (default Kr/10).
+        std::vector<double> cg;
+        {
+          const int Kg = p.refine_guard_degree >= 0 ? p.refine_guard_degree
+                                                    : std::max(1, (Kr + 9) / 10);
+          if (Kg < Kr)
+            cg = build_coeffs(Kg, smin, smax, p.sigma, p.emin, p.emax, p.window_type,
+                              p.jackson != 0);
+        }
         RRWork rr;
         int nlock = 0;
         std::vector<double> lam_lock, res_lock;
@@ -1636,9 +1662,14 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
           } else {
             std::vector<double> th(kz);
             for (int i = 0; i < kz; ++i) th[i] = it == 0 ? lam[zc[i]] : lam_act[i];
-            residual_filter(cx, op, kz, Z.p, ldz, th, cr, smin, smax, Ytmp.p, ldz, rfw);
+            const bool split = it == 0 && !cg.empty() && npre < kz;
+            residual_filter(cx, op, kz, Z.p, ldz, th, cr, smin, smax, Ytmp.p, ldz, rfw,
+                            split ? npre : -1, split ? &cg : nullptr);
           }
-          refine_steps += (long long)Kr * kz;
+          if (it == 0 && !cg.empty() && npre < kz && !fp64_filter)
+            refine_steps += (long long)Kr * npre + (long long)((int)cg.size() - 1) * (kz - npre);
+          else
+            refine_steps += (long long)Kr * kz;
           ++xs.refine_iterations;
           // project out the locked vectors
           if (nlock > 0) {

@@ -37,6 +37,7 @@ mutable struct PolfedParams
     window_type::Cint; jackson::Cint; mmax::Cint; rtol::Cdouble; reorth_passes::Cint
     extra_keep::Cint; seed::UInt64; chunk::Cint; show_progress::Cint; precision::Cint
     refine_guard::Cint; refine_degree::Cint; refine_maxiter::Cint; refine_tol::Cdouble
+    refine_guard_degree::Cint
     PolfedParams() = new()
 end
 

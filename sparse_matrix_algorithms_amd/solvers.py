@@ -401,7 +401,8 @@ def refine_original_rr(A, X: np.ndarray, ctx: Optional[Context] = None):
 # ---------------------------------------------------------------------------
 def _polfed_params(ctx, s, smin, smax, sigma, emin, emax, tol, overextract, refine, window_type,
                    jackson, mmax, rtol, reorth_passes, extra_keep, seed, chunk, show_progress,
-                   precision, refine_guard, refine_degree, refine_maxiter, refine_tol):
+                   precision, refine_guard, refine_degree, refine_maxiter, refine_tol,
+                   refine_guard_degree=None):
     if window_type not in L.WINDOWS:
         raise ValueError(f"Unknown window type: {window_type}")
     p = L.PolfedParams()
@@ -420,6 +421,7 @@ def _polfed_params(ctx, s, smin, smax, sigma, emin, emax, tol, overextract, refi
     p.refine_degree = -1 if refine_degree is None else refine_degree
     p.refine_maxiter = refine_maxiter
     p.refine_tol = -1.0 if refine_tol is None else refine_tol
+    p.refine_guard_degree = -1 if refine_guard_degree is None else refine_guard_degree
     return p
 
 
@@ -431,8 +433,8 @@ def polfed(A, nev: int, degree: int, s: int = 8, smin=None, smax=None, sigma: fl
            show_progress: bool = False, chunk: int = 64, Q1: Optional[np.ndarray] = None,
            precision: str = "fp64", refine_guard: Optional[int] = None,
            refine_degree: Optional[int] = None, refine_maxiter: int = 3,
-           refine_tol: Optional[float] = None, return_stats: bool = False,
-           ctx: Optional[Context] = None, want_X: bool = True):
+           refine_tol: Optional[float] = None, refine_guard_degree: Optional[int] = None,
+           return_stats: bool = False, ctx: Optional[Context] = None, want_X: bool = True):
     """``polfed(A, nev, degree; kwargs...)`` on the GPU.
 
     ``A``: SciPy sparse / :class:`DeviceMatrix` (the sparse method; bounds
@@ -449,7 +451,7 @@ def polfed(A, nev: int, degree: int, s: int = 8, smin=None, smax=None, sigma: fl
     p = _polfed_params(ctx, s, smin, smax, sigma, emin, emax, tol, overextract, refine,
                        window_type, jackson, mmax, rtol, reorth_passes, extra_keep, seed, chunk,
                        show_progress, precision, refine_guard, refine_degree, refine_maxiter,
-                       refine_tol)
+                       refine_tol, refine_guard_degree)
     n = mf.n if mf is not None else dm.n
     q = None
     if Q1 is not None:
@@ -562,7 +564,8 @@ def polfed_slices(A, slices, s: int = 8, smin=None, smax=None, tol: float = 1e-1
                   jackson: bool = True, rtol: float = 1e-12, reorth_passes: int = 1,
                   extra_keep: int = 8, seed: int = 1, precision: str = "fp64",
                   refine_guard: Optional[int] = None, refine_degree: Optional[int] = None,
-                  refine_maxiter: int = 3, refine_tol: Optional[float] = None, use_graph: bool = True,
+                  refine_maxiter: int = 3, refine_tol: Optional[float] = None,
+                  refine_guard_degree: Optional[int] = None, use_graph: bool = True,
                   show_progress: bool = False, want_X: bool = True,
                   ctx: Optional[Context] = None):
     """POLFED over many intervals of one sparse matrix in one batched run.
@@ -580,7 +583,8 @@ def polfed_slices(A, slices, s: int = 8, smin=None, smax=None, tol: float = 1e-1
         raise ValueError("batched slices need a sparse matrix")
     p = _polfed_params(ctx, s, smin, smax, 0.0, -math.inf, math.inf, tol, overextract, refine,
                        window_type, jackson, None, rtol, reorth_passes, extra_keep, seed, 64,
-                       show_progress, precision, refine_guard, refine_degree, refine_maxiter, refine_tol)
+                       show_progress, precision, refine_guard, refine_degree, refine_maxiter, refine_tol,
+                       refine_guard_degree)
     ns = len(slices)
     arr = (L.SliceParams * ns)()
     for i, d in enumerate(slices):

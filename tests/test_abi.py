@@ -70,3 +70,49 @@ def test_no_cpu_fallback_without_library(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(ImportError, match="no CPU fallback"):
         _lib.load()
+
+
+def _header_struct_fields(name):
+    txt = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    body = re.search(r"typedef struct\s+" + name + r"\s*\{([^}]*)\}\s*" + name + r"\s*;",
+                     txt).group(1)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        ctype, rest = decl.split(None, 1)
+        for f in rest.split(","):
+            fields.append((ctype, f.strip()))
+    return fields
+
+
+@pytest.mark.parametrize("cname,pyname", [("smalgo_polfed_params", "PolfedParams"),
+                                          ("smalgo_cheb_params", "ChebParams"),
+                                          ("smalgo_slice", "SliceParams")])
+def test_param_structs_mirror_the_header(cname, pyname):
+    """The ctypes mirrors list the header's fields in order with the same C types."""
+    from sparse_matrix_algorithms_amd import _lib
+    ctypes_of = {"int": C.c_int, "double": C.c_double, "uint64_t": C.c_uint64}
+    want = [(f, ctypes_of[t]) for t, f in _header_struct_fields(cname)]
+    have = [(f, t) for f, t in getattr(_lib, pyname)._fields_]
+    assert have == want
+
+
+def test_default_params_through_abi():
+    """smalgo_*_default_params fill the mirrors with the reference's defaults
+    (polfed.jl:1128, chebyshev_subspace_iteration.jl:667) and the mixed path's."""
+    from sparse_matrix_algorithms_amd import _lib
+    lib = _lib.load()
+    p = _lib.PolfedParams()
+    lib.smalgo_polfed_default_params(C.byref(p))
+    assert (p.s, p.has_bounds, p.sigma, p.tol, p.overextract, p.refine) == (8, 0, 0.0, 1e-10, 2, 1)
+    assert (p.emin, p.emax) == (-math.inf, math.inf)
+    assert (p.jackson, p.mmax, p.rtol, p.reorth_passes, p.extra_keep) == (1, 0, 1e-12, 1, 8)
+    assert (p.seed, p.chunk, p.show_progress, p.precision) == (1, 64, 0, 0)
+    assert (p.refine_guard, p.refine_degree, p.refine_maxiter) == (-1, -1, 3)
+    assert (p.refine_tol, p.refine_guard_degree) == (-1.0, -1)
+    q = _lib.ChebParams()
+    lib.smalgo_cheb_default_params(C.byref(q))
+    assert (q.m0, q.degree, q.maxiter, q.tol, q.res_gate) == (200, 80, 8, 1e-10, 1e-6)
+    assert (q.jackson, q.debug, q.two_hit, q.seed, q.window_type) == (1, 0, 1, 1, 0)

@@ -129,3 +129,34 @@ def test_mixed_refine_residual_form_matches_fp64_filter(ctx, monkeypatch):
     assert st_r["refine_filter_steps"] > 0 and st_d["refine_filter_steps"] > 0
     R = A @ X_r - X_r * lam_r[None, :]
     assert np.max(np.linalg.norm(R, axis=0)) <= 2e-10 * anorm
+
+
+@pytest.mark.parametrize("guard_degree", [None, 1, 700])
+def test_mixed_refine_guard_degree(ctx, guard_degree):
+    """The guard vectors of the first refinement iteration only widen the
+    Rayleigh–Ritz subspace: at the default (a tenth of the refinement degree),
+    at degree 1 and at the full degree the mixed path returns the dense-eigh
+    eigenvalues, and the lower guard degree runs fewer filter steps."""
+    import sparse_matrix_algorithms_amd as sm
+    A = anderson3d(12, 16.5, 8)
+    n = A.shape[0]
+    ev = np.linalg.eigh(A.toarray())[0]
+    nev = 60
+    emin, emax = _window(ev, nev)
+    exact = ev[(ev >= emin) & (ev <= emax)]
+    smin, smax = P.gershgorin_bounds(A)
+    anorm = max(abs(smin), abs(smax))
+    K = 700
+    kw = dict(s=16, smin=smin, smax=smax, emin=emin, emax=emax, mmax=math.ceil(3 * nev / 16),
+              precision="fp32", return_stats=True, ctx=ctx, Q1=gaussian_block(2, n, 16))
+    lam, X, res, st = sm.polfed(A, nev, K, refine_guard_degree=guard_degree, **kw)
+    assert len(lam) == nev
+    assert np.max(np.abs(np.sort(lam) - exact)) <= 1e-10 * anorm
+    assert res.max() <= 1e-10 * anorm
+    if st["refine_iterations"] == 1:
+        kz = st["refine_filter_steps"] / K  # columns at the full degree if none was lowered
+        assert kz >= nev
+        if guard_degree == 700:
+            assert st["refine_filter_steps"] % K == 0
+        else:
+            assert st["refine_filter_steps"] % K != 0
