@@ -6,6 +6,10 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
 
 #include "../../include/smalgo.h"
 #include "smalgo_coeffs.h"
@@ -681,6 +685,89 @@ int64_t smalgo_result_n(const smalgo_result* h) {
   return r ? r->n : -1;
 }
 
+}  // extern "C"
+
+namespace {
+// Pinned staging for large result downloads, one per device, kept for the life
+// of the process: two 64 MiB blocks and a copy stream.  Chunk c+1 crosses PCIe
+// into one block while host threads copy chunk c out of the other into the
+// caller's (pageable) array.
+struct Download {
+  hipStream_t st = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  double* buf[2] = {nullptr, nullptr};
+  size_t cap = 0;  // doubles per block
+};
+std::mutex g_dl_mu;
+std::map<int, Download> g_dl;
+constexpr size_t kDownloadChunkBytes = size_t(64) << 20;
+
+hipError_t download_setup(Download& d, size_t doubles) {
+  hipError_t e = hipSuccess;
+  if (!d.st && (e = hipStreamCreateWithFlags(&d.st, hipStreamNonBlocking)) != hipSuccess) return e;
+  for (auto& ev : d.ev)
+    if (!ev && (e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return e;
+  if (d.cap >= doubles) return hipSuccess;
+  for (auto& b : d.buf) {
+    if (b) (void)hipHostFree(b);
+    b = nullptr;
+  }
+  d.cap = 0;
+  for (auto& b : d.buf)
+    if ((e = hipHostMalloc((void**)&b, doubles * sizeof(double), hipHostMallocDefault)) !=
+        hipSuccess)
+      return e;
+  d.cap = doubles;
+  return hipSuccess;
+}
+
+// columns [c0, c0+nc) of a column-major n-row block from pinned src into X
+void host_copy_cols(const double* src, size_t n, size_t nc, double* X, size_t ldx, size_t c0) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nthr = std::min<size_t>(std::min<size_t>(8, hw), std::max<size_t>(1, nc * n >> 20));
+  auto part = [&](size_t t) {
+    if (ldx == n) {  // one contiguous range: split by bytes
+      const size_t tot = n * nc, lo = tot * t / nthr, hi = tot * (t + 1) / nthr;
+      std::memcpy(X + c0 * n + lo, src + lo, (hi - lo) * sizeof(double));
+    } else {
+      for (size_t c = nc * t / nthr; c < nc * (t + 1) / nthr; ++c)
+        std::memcpy(X + (c0 + c) * ldx, src + c * n, n * sizeof(double));
+    }
+  };
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nthr; ++t) th.emplace_back(part, t);
+  part(0);
+  for (auto& x : th) x.join();
+}
+
+int download_pinned(const Result* r, double* X, int64_t ldx, size_t k) {
+  const size_t n = (size_t)r->n;
+  const size_t cpc = std::max<size_t>(1, kDownloadChunkBytes / (n * sizeof(double)));
+  std::lock_guard<std::mutex> lock(g_dl_mu);
+  Download& d = g_dl[r->device];
+  if (download_setup(d, cpc * n) != hipSuccess) return SMALGO_ERR_HIP;
+  const size_t nch = (k + cpc - 1) / cpc;
+  auto issue = [&](size_t c) {
+    const size_t c0 = c * cpc, nc = std::min(cpc, k - c0);
+    hipError_t e = hipMemcpyAsync(d.buf[c & 1], r->Xdev + c0 * n, nc * n * sizeof(double),
+                                  hipMemcpyDeviceToHost, d.st);
+    if (e == hipSuccess) e = hipEventRecord(d.ev[c & 1], d.st);
+    return e;
+  };
+  if (issue(0) != hipSuccess) return SMALGO_ERR_HIP;
+  for (size_t c = 0; c < nch; ++c) {
+    // block (c+1)&1 last held chunk c-1, whose host copy finished in the previous pass
+    if (c + 1 < nch && issue(c + 1) != hipSuccess) return SMALGO_ERR_HIP;
+    if (hipEventSynchronize(d.ev[c & 1]) != hipSuccess) return SMALGO_ERR_HIP;
+    const size_t c0 = c * cpc, nc = std::min(cpc, k - c0);
+    host_copy_cols(d.buf[c & 1], n, nc, X, (size_t)ldx, c0);
+  }
+  return SMALGO_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int smalgo_result_copy(const smalgo_result* h, double* lam, double* X, int64_t ldx, double* res) {
   const Result* r = res_of(h);
   if (!r) return SMALGO_ERR_ARG;
@@ -690,9 +777,11 @@ int smalgo_result_copy(const smalgo_result* h, double* lam, double* X, int64_t l
   if (X && k > 0) {
     if (ldx < r->n) return SMALGO_ERR_ARG;
     if (hipSetDevice(r->device) != hipSuccess) return SMALGO_ERR_HIP;
+    const size_t bytes = sizeof(double) * (size_t)r->n * k;
+    if (bytes >= 2 * kDownloadChunkBytes) return download_pinned(r, X, ldx, k);
     hipError_t e;
     if (ldx == r->n)
-      e = hipMemcpy(X, r->Xdev, sizeof(double) * (size_t)r->n * k, hipMemcpyDeviceToHost);
+      e = hipMemcpy(X, r->Xdev, bytes, hipMemcpyDeviceToHost);
     else
       e = hipMemcpy2D(X, sizeof(double) * ldx, r->Xdev, sizeof(double) * r->n,
                       sizeof(double) * r->n, k, hipMemcpyDeviceToHost);

@@ -74,3 +74,39 @@ def test_feast_style_loop_with_host_solves(ctx):
     anorm = np.abs(ev).max()
     assert inside.sum() == len(exact)
     assert np.max(np.abs(np.sort(lam[inside]) - exact)) <= 1e-10 * anorm
+
+
+def test_large_result_download_paths(ctx):
+    """Results of 128 MiB and more come back through pinned, pipelined chunks
+    and host threads; the copy must equal the plain one column for column, for
+    a packed and for a padded leading dimension (n=110,592, 180 columns =
+    159 MB, three 64 MiB chunks)."""
+    import ctypes as C
+    import sparse_matrix_algorithms_amd as sm
+    from sparse_matrix_algorithms_amd import _lib as L
+    A = anderson3d(48, 16.5, 48)
+    n, k = A.shape[0], 180
+    dm = sm.DeviceMatrix(A, ctx)
+    Y = np.asfortranarray(gaussian_block(9, n, k))
+    h = C.c_void_p()
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    ctx.check(ctx.lib.smalgo_subspace_rr(ctx.h, dm.h, L.MATBLOCK_FN(), None, n, k, dp(Y), n, None,
+                                         n, C.byref(h)))
+    try:
+        assert ctx.lib.smalgo_result_count(h) == k
+        lam, res = np.zeros(k), np.zeros(k)
+        X = np.zeros((n, k), order="F")
+        ctx.check(ctx.lib.smalgo_result_copy(h, dp(lam), dp(X), n, dp(res)))
+        ldx = n + 5
+        Xp = np.full((ldx, k), np.nan, order="F")
+        ctx.check(ctx.lib.smalgo_result_copy(h, None, dp(Xp), ldx, None))
+        assert np.array_equal(Xp[:n], X)
+        assert np.isnan(Xp[n:]).all()  # padding rows untouched
+    finally:
+        ctx.lib.smalgo_result_free(h)
+        dm.free()
+    # the vectors are the Rayleigh–Ritz vectors of span(Y): orthonormal, with
+    # the reported residuals
+    assert np.max(np.abs(X.T @ X - np.eye(k))) <= 1e-10
+    R = A @ X - X * lam[None, :]
+    np.testing.assert_allclose(np.linalg.norm(R, axis=0), res, rtol=1e-6, atol=1e-12)
