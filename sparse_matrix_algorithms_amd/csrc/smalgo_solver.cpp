@@ -1624,8 +1624,8 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
           if (!used[jj]) zc.push_back(jj);
         int kz = (int)zc.size();
         int ldz = round_up(kz, 4);
-        DBuf<double> Z((size_t)n * ldz), Zb0((size_t)n * ldz), Zb1((size_t)n * ldz),
-            Lk((size_t)n * ldz), Ytmp((size_t)n * ldz), Zt((size_t)n * ldz);
+        DBuf<double> Z((size_t)n * ldz), Lk((size_t)n * ldz), Ytmp((size_t)n * ldz);
+        DBuf<double> Zb0, Zb1;  // ping-pong blocks of the FP64 refinement filter only
         DBuf<int> dz = upload_idx(cx, zc);
         gather_cols_rm<double>(cx.stream, n, kz, X.p, ldu, dz.p, Z.p, ldz);
         const int Kr = p.refine_degree >= 0 ? p.refine_degree : degree;
@@ -1653,6 +1653,10 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
         if (op.is_callback()) pb2.alloc((size_t)n * ldz);
         ResidualFilterWork rfw;
         const bool fp64_filter = refine_fp64_env() || op.is_callback();
+        if (fp64_filter) {
+          Zb0.alloc((size_t)n * ldz);
+          Zb1.alloc((size_t)n * ldz);
+        }
         for (int it = 0; it < p.refine_maxiter && kz > 0; ++it) {
           // Y = p_r(Â) Z: in residual form with an FP32 filter (SMALGO_REFINE_FP64=1:
           // the FP64 filter on Z itself)
