@@ -116,7 +116,9 @@ end
 # int (*)(void* user, int64 n, int64 b, const double* X, int64 ldx, double* Y, int64 ldy)
 function _matblock_tramp(user::Ptr{Cvoid}, n::Int64, b::Int64, X::Ptr{Cdouble}, ldx::Int64,
                          Y::Ptr{Cdouble}, ldy::Int64)::Cint
-    op = unsafe_pointer_to_objref(user)::MatrixFreeRealSymOp
+    # `user` points at a Ref box: the operator itself is immutable, and
+    # pointer_from_objref needs a mutable object
+    op = (unsafe_pointer_to_objref(user)::Base.RefValue{Any})[]::MatrixFreeRealSymOp
     try
         Xa = view(unsafe_wrap(Matrix{Float64}, X, (ldx, b)), 1:n, :)
         Ya = view(unsafe_wrap(Matrix{Float64}, Y, (ldy, b)), 1:n, :)
@@ -243,11 +245,12 @@ function apply_poly_block!(Y::AbstractMatrix{Float64}, A::MatrixFreeRealSymOp,
     n, b = size(Q)
     (size(Y) == (n, b) && n == A.n) || throw(DimensionMismatch("Y, Q and A sizes differ"))
     Qc = Matrix{Float64}(Q); Yc = Matrix{Float64}(undef, n, b); co = Vector{Float64}(coeffs)
-    GC.@preserve A Qc Yc co begin
+    box = Ref{Any}(A)
+    GC.@preserve box Qc Yc co begin
         check(ccall((:smalgo_apply_poly_block_mf, libsmalgo), Cint,
                     (Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cdouble}, Int64,
                      Ptr{Cdouble}, Cint, Cdouble, Cdouble, Ptr{Cdouble}, Int64),
-                    context().h, n, tramp(), pointer_from_objref(A), b, Qc, n, co,
+                    context().h, n, tramp(), pointer_from_objref(box), b, Qc, n, co,
                     length(co) - 1, smin, smax, Yc, n))
     end
     copyto!(Y, Yc)
@@ -300,13 +303,14 @@ function _polfed(dev, op, n::Int, nev::Int, degree::Int; s::Int=8, smin=nothing,
     Random.seed!(seed)
     Q1 = randn(n, s)                       # polfed.jl:1133-1134
     h = Ref{Ptr{Cvoid}}(C_NULL)
-    GC.@preserve op Q1 begin
+    box = Ref{Any}(op)
+    GC.@preserve box Q1 begin
         rc = ccall((:smalgo_polfed, libsmalgo), Cint,
                    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Cint, Cint,
                     Ref{PolfedParams}, Ptr{Cdouble}, Ptr{Ptr{Cvoid}}),
                    context().h, dev === nothing ? C_NULL : dev.h,
                    op === nothing ? C_NULL : tramp(),
-                   op === nothing ? C_NULL : pointer_from_objref(op), n, nev, degree, p, Q1, h)
+                   op === nothing ? C_NULL : pointer_from_objref(box), n, nev, degree, p, Q1, h)
     end
     check(rc)
     return take_result(h[])
@@ -404,13 +408,14 @@ function _cheb(dev, op, n::Int, emin::Float64, emax::Float64; smin=nothing, smax
     Random.seed!(seed)
     Q0 = randn(n, min(m0, n))              # chebyshev_subspace_iteration.jl:701-702
     h = Ref{Ptr{Cvoid}}(C_NULL)
-    GC.@preserve op Q0 begin
+    box = Ref{Any}(op)
+    GC.@preserve box Q0 begin
         rc = ccall((:smalgo_chebyshev, libsmalgo), Cint,
                    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Cdouble, Cdouble,
                     Ref{ChebParams}, Ptr{Cdouble}, Ptr{Ptr{Cvoid}}),
                    context().h, dev === nothing ? C_NULL : dev.h,
                    op === nothing ? C_NULL : tramp(),
-                   op === nothing ? C_NULL : pointer_from_objref(op), n, emin, emax, p, Q0, h)
+                   op === nothing ? C_NULL : pointer_from_objref(box), n, emin, emax, p, Q0, h)
     end
     check(rc)
     return take_result(h[])
