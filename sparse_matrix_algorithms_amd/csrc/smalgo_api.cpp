@@ -8,6 +8,7 @@
 #include <limits>
 #include <map>
 #include <mutex>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -735,7 +736,13 @@ void host_copy_cols(const double* src, size_t n, size_t nc, double* X, size_t ld
     }
   };
   std::vector<std::thread> th;
-  for (size_t t = 1; t < nthr; ++t) th.emplace_back(part, t);
+  size_t t = 1;
+  try {
+    for (; t < nthr; ++t) th.emplace_back(part, t);
+  } catch (const std::system_error&) {
+    // no more threads available: this thread copies the remaining parts
+  }
+  for (size_t u = t; u < nthr; ++u) part(u);
   part(0);
   for (auto& x : th) x.join();
 }
@@ -778,7 +785,13 @@ int smalgo_result_copy(const smalgo_result* h, double* lam, double* X, int64_t l
     if (ldx < r->n) return SMALGO_ERR_ARG;
     if (hipSetDevice(r->device) != hipSuccess) return SMALGO_ERR_HIP;
     const size_t bytes = sizeof(double) * (size_t)r->n * k;
-    if (bytes >= 2 * kDownloadChunkBytes) return download_pinned(r, X, ldx, k);
+    if (bytes >= 2 * kDownloadChunkBytes) {
+      try {
+        return download_pinned(r, X, ldx, k);
+      } catch (const std::exception&) {
+        return SMALGO_ERR_INTERNAL;
+      }
+    }
     hipError_t e;
     if (ldx == r->n)
       e = hipMemcpy(X, r->Xdev, bytes, hipMemcpyDeviceToHost);
