@@ -59,6 +59,12 @@ void smalgo_ctx_destroy(smalgo_ctx* ctx);
 const char* smalgo_last_error(const smalgo_ctx* ctx);
 /* Library version string, e.g. "smalgo 0.1 gfx950". */
 const char* smalgo_version(void);
+/* v as Julia's print(::Float64) writes it, the form of every float in the
+ * progress line of polfed.jl:991-1000 (println of Float64 values): shortest
+ * round-trip digits, fixed notation for 1e-4 <= |v| < 1e6, otherwise
+ * d.ddde<exp> with no zero-padded exponent ("1.0e-5", "3.1117078311317e-14").
+ * buf receives a NUL-terminated string of at most len-1 characters.  No GPU. */
+int smalgo_julia_float_string(double v, char* buf, int64_t len);
 
 /* ---- sparse matrix ------------------------------------------------------
  * Upload a compressed sparse matrix.  ptr has n+1 entries, idx/val have nnz.

@@ -332,6 +332,37 @@ def extract_pairs(Qhist, Ablocks, Bblocks, A, nev: int, emin: float, emax: float
     return lamf[pe], Xf[:, pe], rf[pe], thsel
 
 
+def julia_float(v: float) -> str:
+    """``v`` as Julia's ``print(::Float64)`` writes it (what ``println`` puts in
+    the progress line, ``polfed.jl:999``): the shortest digit string that
+    round-trips, ``value = 0.d1…dk × 10^pt``; fixed notation when
+    ``-4 < pt <= 6``, otherwise ``d1.d2…dk e<pt-1>`` with no ``+`` and no zero
+    padding; a single digit keeps a trailing ``.0`` (``1.0e-5``, ``0.0001``,
+    ``123456.0``, ``1.234567e6``)."""
+    import decimal
+    v = float(v)
+    if math.isnan(v):
+        return "NaN"
+    if math.isinf(v):
+        return "Inf" if v > 0 else "-Inf"
+    if v == 0.0:
+        return "-0.0" if math.copysign(1.0, v) < 0 else "0.0"
+    # repr gives the shortest round-trip digits (David Gay's algorithm)
+    sign, dig, e = decimal.Decimal(repr(v)).as_tuple()
+    digits = "".join(map(str, dig)).rstrip("0") or "0"
+    e += len(dig) - len(digits)          # value = digits × 10^e
+    k = len(digits)
+    pt = e + k
+    out = "-" if sign else ""
+    if -4 < pt <= 6:
+        if pt <= 0:
+            return out + "0." + "0" * (-pt) + digits
+        if pt >= k:
+            return out + digits + "0" * (pt - k) + ".0"
+        return out + digits[:pt] + "." + digits[pt:]
+    return out + digits[0] + "." + (digits[1:] or "0") + "e" + str(pt - 1)
+
+
 def progress_line(m: int, lam, res) -> str:
     """``m=…  inside=…  r90=…  r95=…  maxres=…`` (``polfed.jl:991-1000``)."""
     if len(res) == 0:
@@ -342,8 +373,8 @@ def progress_line(m: int, lam, res) -> str:
     def _ridx(f):  # Julia round(Int, x): round half to even, 1-based clamp
         return min(max(int(np.round(f * L)), 1), L) - 1
 
-    return (f"m={m}  inside={len(lam)}  r90={rs[_ridx(0.9)]!r}  r95={rs[_ridx(0.95)]!r}"
-            f"  maxres={rs[-1]!r}")
+    return (f"m={m}  inside={len(lam)}  r90={julia_float(rs[_ridx(0.9)])}"
+            f"  r95={julia_float(rs[_ridx(0.95)])}  maxres={julia_float(rs[-1])}")
 
 
 # ----------------------------------------------------------------------------

@@ -63,6 +63,7 @@ SIGNATURES = [
     ("smalgo_ctx_destroy", None, [_P]),
     ("smalgo_last_error", C.c_char_p, [_P]),
     ("smalgo_version", C.c_char_p, []),
+    ("smalgo_julia_float_string", C.c_int, [C.c_double, C.c_char_p, C.c_int64]),
     ("smalgo_csr_upload", C.c_int, [_P, C.c_int64, _I64P, _I64P, _DP, C.c_int, C.POINTER(_P)]),
     ("smalgo_mat_free", None, [_P]),
     ("smalgo_mat_n", C.c_int64, [_P]),

@@ -64,6 +64,18 @@ extern "C" {
 
 const char* smalgo_version(void) { return "smalgo 0.1 gfx950"; }
 
+int smalgo_julia_float_string(double v, char* buf, int64_t len) {
+  if (!buf || len <= 0) return SMALGO_ERR_ARG;
+  try {
+    const std::string s = format_julia_float(v);
+    if ((int64_t)s.size() >= len) return SMALGO_ERR_ARG;
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    return SMALGO_OK;
+  } catch (const std::exception&) {
+    return SMALGO_ERR_INTERNAL;
+  }
+}
+
 int smalgo_ctx_create(int device, smalgo_ctx** out) {
   if (!out) return SMALGO_ERR_ARG;
   *out = nullptr;
@@ -109,13 +121,16 @@ int smalgo_csr_upload(smalgo_ctx* ctx, int64_t n, const int64_t* ptr, const int6
     if (nnz < 0 || nnz >= (int64_t)std::numeric_limits<int>::max())
       throw Error(-1, "nnz out of range");
     if (nnz > 0 && (!idx || !val)) throw Error(-1, "idx/val are null");
+    // the whole pointer array first, in int64: every offset in [0, nnz] and
+    // nondecreasing, before any idx/val element is touched
+    for (int64_t i = 0; i < n; ++i)
+      if (ptr[i + 1] < ptr[i]) throw Error(-1, "ptr must be nondecreasing");
     std::vector<int> rp(n + 1), ci(nnz);
     std::vector<double> v64(val, val + nnz);
     std::vector<float> v32(nnz);
     std::vector<double> dg(n, 0.0), rs(n, 0.0);
     for (int64_t i = 0; i <= n; ++i) rp[i] = (int)(ptr[i] - ptr[0]);
     for (int64_t i = 0; i < n; ++i) {
-      if (rp[i + 1] < rp[i]) throw Error(-1, "ptr must be nondecreasing");
       for (int p = rp[i]; p < rp[i + 1]; ++p) {
         const int64_t j = idx[p] - index_base;
         if (j < 0 || j >= n) throw Error(-1, "column index out of range");
@@ -656,14 +671,21 @@ int smalgo_csr_upload_file(smalgo_ctx* ctx, const char* path, smalgo_mat** out) 
     ctx->err = g_io_err;
     return rc;
   }
-  std::vector<int64_t> rp(n + 1), ci(nnz);
-  std::vector<double> v(nnz);
-  rc = smalgo_csr_load(path, rp.data(), ci.data(), v.data());
-  if (rc != SMALGO_OK) {
-    ctx->err = g_io_err;
-    return rc;
+  // the header was checked against the file size; allocation failures still
+  // return an error code instead of unwinding through extern "C"
+  try {
+    std::vector<int64_t> rp(n + 1), ci(nnz);
+    std::vector<double> v(nnz);
+    rc = smalgo_csr_load(path, rp.data(), ci.data(), v.data());
+    if (rc != SMALGO_OK) {
+      ctx->err = g_io_err;
+      return rc;
+    }
+    return smalgo_csr_upload(ctx, n, rp.data(), ci.data(), v.data(), 0, out);
+  } catch (const std::exception& e) {
+    ctx->err = std::string("loading ") + path + ": " + e.what();
+    return SMALGO_ERR_INTERNAL;
   }
-  return smalgo_csr_upload(ctx, n, rp.data(), ci.data(), v.data(), 0, out);
 }
 
 int smalgo_eig_save(const char* path, int64_t count, const double* lam, const double* res) {

@@ -105,6 +105,14 @@ static CsrHeader csr_header(File& f) {
   f.read(&h, sizeof h);
   if (std::memcmp(h.magic, kCsrMagic, 16) != 0) throw Error(-1, "not a CSR fixture");
   if (h.n < 0 || h.nnz < 0 || h.index_base != 0) throw Error(-1, "bad CSR header");
+  // the payload the header announces must be in the file: 8 (n+1) + 16 nnz bytes
+  const long long limit = (long long)1 << 60;
+  if (h.n > limit / 64 || h.nnz > limit / 64) throw Error(-1, "bad CSR header");
+  const long long need = 64 + 8 * (h.n + 1) + 16 * h.nnz;
+  if (std::fseek(f.f, 0, SEEK_END) != 0) throw Error(-1, "cannot size the file");
+  const long long size = std::ftell(f.f);
+  if (size < need) throw Error(-1, "file is truncated");
+  if (std::fseek(f.f, (long)sizeof h, SEEK_SET) != 0) throw Error(-1, "cannot seek in the file");
   return h;
 }
 

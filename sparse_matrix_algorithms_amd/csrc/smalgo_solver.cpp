@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <limits>
+#include <memory>
 #include <numeric>
 
 #include "smalgo_coeffs.h"
@@ -565,14 +566,55 @@ static int cholqr(Ctx& cx, Dense& dn, OrthWork& ow, int n, int b, const T* W, in
 // ---------------------------------------------------------------------------
 // formatting of the progress line (polfed.jl:991-1000)
 // ---------------------------------------------------------------------------
+// Julia's print(::Float64): the shortest digit string that round-trips
+// (d_1 … d_k, value = 0.d_1…d_k · 10^pt), written in fixed notation when
+// -4 < pt <= 6 and otherwise as d_1.d_2…d_k e<exp> with exp = pt - 1, no
+// '+' and no zero padding ("1.0e-5", "1.0927391319480254e-9", "0.0001",
+// "123456.0", "1.234567e6").  A single digit keeps a trailing ".0".
 std::string format_julia_float(double v) {
-  if (std::isinf(v)) return v > 0 ? "Inf" : "-Inf";
   if (std::isnan(v)) return "NaN";
+  if (std::isinf(v)) return v > 0 ? "Inf" : "-Inf";
+  if (v == 0.0) return std::signbit(v) ? "-0.0" : "0.0";
   char buf[64];
-  auto r = std::to_chars(buf, buf + sizeof(buf), v);
+  // shortest round-trip digits in scientific form: [-]d[.ddd]e±XX
+  auto r = std::to_chars(buf, buf + sizeof(buf), v, std::chars_format::scientific);
   std::string s(buf, r.ptr);
-  if (s.find_first_of(".eEn") == std::string::npos) s += ".0";
-  return s;
+  std::string out;
+  size_t i = 0;
+  if (s[0] == '-') {
+    out += '-';
+    i = 1;
+  }
+  const size_t epos = s.find('e');
+  std::string digits;
+  for (size_t j = i; j < epos; ++j)
+    if (s[j] != '.') digits += s[j];
+  while (digits.size() > 1 && digits.back() == '0') digits.pop_back();
+  const int e10 = std::atoi(s.c_str() + epos + 1);  // value = d.ddd · 10^e10
+  const int k = (int)digits.size();
+  const int pt = e10 + 1;
+  if (-4 < pt && pt <= 6) {
+    if (pt <= 0) {
+      out += "0.";
+      out.append((size_t)(-pt), '0');
+      out += digits;
+    } else if (pt >= k) {
+      out += digits;
+      out.append((size_t)(pt - k), '0');
+      out += ".0";
+    } else {
+      out += digits.substr(0, pt);
+      out += '.';
+      out += digits.substr(pt);
+    }
+    return out;
+  }
+  out += digits[0];
+  out += '.';
+  out += k > 1 ? digits.substr(1) : std::string("0");
+  out += 'e';
+  out += std::to_string(e10);
+  return out;
 }
 
 std::string progress_line(int m, const std::vector<double>& lam, const std::vector<double>& res) {
@@ -792,18 +834,19 @@ void residuals_host(Ctx& cx, const Op& op, int k, const double* X, int ldx, cons
 Result* subspace_rr_host(Ctx& cx, const Op& op, int k, const double* Y, int ldy, double* Qh,
                          int ldqh) {
   const int n = op.n;
-  Result* out = new Result();
+  std::unique_ptr<Result> out(new Result());
   out->n = n;
   out->device = cx.device;
-  try {
-    if (k == 0) return out;
+  {
+    if (k == 0) return out.release();
     const int ld = round_up(k, 4);
     Dense dn(cx);
     OrthWork ow;
     DBuf<double> Yd((size_t)n * ld), Q((size_t)n * ld), T((size_t)n * ld), AX((size_t)n * ld),
         X((size_t)n * ld), Aq((size_t)k * k), wv(k), ev(k), resd(k);
     upload_cm<double>(cx, n, k, Y, ldy, Yd.p, ld);
-    cholqr<double>(cx, dn, ow, n, k, Yd.p, ld, Q.p, ld, T.p, ld, 0.0, nullptr, true);
+    if (cholqr<double>(cx, dn, ow, n, k, Yd.p, ld, Q.p, ld, T.p, ld, 0.0, nullptr, true) == 0)
+      throw Error(-4, "thin_orth: Cholesky QR of Y failed (zero or non-finite block)");
     apply_A<double>(cx, op, k, Q.p, ld, AX.p, ld);
     DBuf<double> gw(gemm_tn_work_elems(n, k, k));
     gemm_tn<double, double, double>(cx.stream, n, k, k, 1.0, Q.p, ld, AX.p, ld, 0.0, Aq.p, k, gw.p,
@@ -826,11 +869,8 @@ Result* subspace_rr_host(Ctx& cx, const Op& op, int k, const double* Y, int ldy,
     if (Qh) download_cm<double>(cx, n, k, Q.p, ld, Qh, ldqh);
     SMALGO_HIP(hipStreamSynchronize(cx.stream));
     out->line = "subspace_rr k=" + std::to_string(k);
-  } catch (...) {
-    delete out;
-    throw;
   }
-  return out;
+  return out.release();
 }
 
 Result* refine_rr_host(Ctx& cx, const Op& op, int k, const double* X, int ldx) {
@@ -841,7 +881,7 @@ Result* refine_rr_host(Ctx& cx, const Op& op, int k, const double* X, int ldx) {
   RRWork w;
   DBuf<double> Y;
   int ldy = 4;
-  Result* out = new Result();
+  std::unique_ptr<Result> out(new Result());
   out->n = n;
   out->device = cx.device;
   const int kq = refine_rr(cx, dn, op, n, k, dX.p, lk, w, Y, ldy, out->lam, out->res);
@@ -850,7 +890,7 @@ Result* refine_rr_host(Ctx& cx, const Op& op, int k, const double* X, int ldx) {
     rm_to_cm<double>(cx.stream, n, kq, Y.p, ldy, nullptr, out->Xdev, n);
     SMALGO_HIP(hipStreamSynchronize(cx.stream));
   }
-  return out;
+  return out.release();
 }
 
 // ---------------------------------------------------------------------------
@@ -1533,7 +1573,7 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
   int locked_total = 0;
   double gemm_flops = 0;
   size_t hx = tm.begin("extract");
-  Result* out = new Result();
+  std::unique_ptr<Result> out(new Result());
   out->n = n;
   out->device = cx.device;
   std::vector<int> rowmap;
@@ -1850,7 +1890,7 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
   xs.refine_steps += refine_steps;
   xs.locked += locked_total;
   xs.gemm_flops += gemm_flops;
-  return out;
+  return out.release();
 }
 
 // ---------------------------------------------------------------------------
@@ -1908,7 +1948,7 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
     tm.end(ho);
   }
   ExtractStats xs;
-  Result* out = polfed_extract<T>(cx, dn, tm, op, B, nev, degree, smin, smax, p, xs);
+  std::unique_ptr<Result> out(polfed_extract<T>(cx, dn, tm, op, B, nev, degree, smin, smax, p, xs));
   auto ph = tm.resolve();
   const int nb = (int)B.ranks.size();
   out->stats["t_total"] = std::chrono::duration<double>(clk::now() - t0).count();
@@ -1935,7 +1975,7 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
     bytes += (double)degree * ((double)(op.A ? op.A->nnz : 0) * (sizeof(T) + 4) + 4.0 * (n + 1) +
                                4.0 * n * B.ranks[j] * sizeof(T));
   out->stats["spmm_bytes"] = bytes;
-  return out;
+  return out.release();
 }
 
 Result* polfed_solve(Ctx& cx, const Op& op, int nev, int degree, const smalgo_polfed_params& p,
@@ -1966,7 +2006,7 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
     smin = op.A->gersh_min;
     smax = op.A->gersh_max;
   }
-  Result* out = new Result();
+  std::unique_ptr<Result> out(new Result());
   out->n = n;
   out->device = cx.device;
   if (!(emin >= smin && emax <= smax)) {
@@ -1979,7 +2019,6 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
   const double c = 0.5 * (smin + smax), d = 0.5 * (smax - smin);
   const double a = (emin - c) / d, b = (emax - c) / d;
   if (!(-1.0 <= a && a < b && b <= 1.0)) {
-    delete out;
     char buf[256];
     std::snprintf(buf, sizeof buf, "mapped interval [%s,%s] not inside [-1,1]%s",
                   format_julia_float(a).c_str(), format_julia_float(b).c_str(),
@@ -1995,7 +2034,6 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
       coeffs = cheb_smooth_window_coeffs(a, b, p.degree, p.jackson != 0, 4096, 0.02);
       break;
     default:
-      delete out;
       throw Error(-1, "Unknown window type, possible are :step, :delta, and :smooth");
   }
   const int ld = round_up(m0, 4);
@@ -2031,8 +2069,12 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
     tm.end(hf);
     filter_steps += p.degree;
     size_t ho = tm.begin("orth");
-    // Q .= Y; thin_orth!(Q)  — all min(n,m0) columns kept (:733-734)
-    cholqr<double>(cx, dn, ow, n, m0, Y.p, ld, Q.p, ld, B0.p, ld, 0.0, nullptr, true);
+    // Q .= Y; thin_orth!(Q)  — all min(n,m0) columns kept (:733-734).  With
+    // rtol = 0 a zero rank means the Cholesky QR gave up (a zero or non-finite
+    // block, e.g. NaN from a host operator): Q would be stale, so stop here.
+    if (cholqr<double>(cx, dn, ow, n, m0, Y.p, ld, Q.p, ld, B0.p, ld, 0.0, nullptr, true) == 0)
+      throw Error(-4, "thin_orth: Cholesky QR of the filtered block failed (zero or "
+                      "non-finite block at iteration " + std::to_string(it) + ")");
     tm.end(ho);
     size_t hr = tm.begin("rr");
     apply_A<double>(cx, op, m0, Q.p, ld, AX.p, ld);                       // R = A Q
@@ -2113,7 +2155,7 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
   char buf[64];
   std::snprintf(buf, sizeof buf, "it=%d inside=%zu", its, out->lam.size());
   out->line = buf;
-  return out;
+  return out.release();
 }
 
 static int slice_pw_env() {
@@ -2283,6 +2325,7 @@ static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
   }
   const int pw_used = sf.max_width();
   std::vector<Result*> out(ns, nullptr);
+  std::map<std::string, double> ph_all;
   try {
     for (int i = 0; i < ns; ++i) {
       smalgo_polfed_params pp = p;
@@ -2299,11 +2342,12 @@ static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
       out[i]->stats["locked"] = xs.locked;
       out[i]->stats["gemm_flops"] = xs.gemm_flops;
     }
+    ph_all = tm.resolve();
   } catch (...) {
     for (Result* r : out) delete r;
     throw;
   }
-  auto ph = tm.resolve();
+  auto& ph = ph_all;
   const double ttot = std::chrono::duration<double>(clk::now() - t0).count();
   for (int i = 0; i < ns; ++i) {
     Result* r = out[i];

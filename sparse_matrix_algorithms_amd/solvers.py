@@ -302,9 +302,11 @@ def _take_result(ctx: Context, h, want_X: bool = True):
 def orth_against(W: np.ndarray, Q: np.ndarray, precision: str = "fp64",
                  ctx: Optional[Context] = None) -> np.ndarray:
     """``orth_against!(W,Q)``: returns ``W − Q(QᵀW)`` (``polfed.jl:605-609``)."""
-    ctx = ctx or default_context()
     Wf = np.array(W, dtype=np.float64, order="F")
     Qf = np.asfortranarray(Q, dtype=np.float64)
+    if Wf.ndim != 2 or Qf.ndim != 2 or Qf.shape[0] != Wf.shape[0]:
+        raise ValueError(f"Q must have as many rows as W ({Wf.shape[0]})")
+    ctx = ctx or default_context()
     n, b = Wf.shape
     ctx.check(ctx.lib.smalgo_orth_against(ctx.h, n, b, _dptr(Wf), n, Qf.shape[1], _dptr(Qf), n,
                                           L.PRECISIONS[precision]))
@@ -399,6 +401,25 @@ def refine_original_rr(A, X: np.ndarray, ctx: Optional[Context] = None):
 # ---------------------------------------------------------------------------
 # POLFED (polfed.jl:1128-1186)
 # ---------------------------------------------------------------------------
+def _start_block(Q0, n: int, m0: int):
+    """The caller's ``randn(n, m0)`` replacement (``chebyshev_subspace_iteration.jl:701-702``)
+    as an ``n × min(m0, n)`` column-major array; the library reads exactly that many
+    values, so a smaller array is an error here, before anything reaches the device."""
+    if Q0 is None:
+        return None
+    Q0 = np.asarray(Q0)
+    k = min(m0, n)
+    if Q0.ndim != 2 or Q0.shape[0] != n or Q0.shape[1] < k:
+        raise ValueError(f"Q0 must be {n} x (at least) {k}, got {Q0.shape}")
+    return np.asfortranarray(Q0[:, :k], dtype=np.float64)
+
+
+def _n_of(A) -> int:
+    if isinstance(A, (MatrixFreeRealSymOp, DeviceMatrix)):
+        return int(A.n)
+    return int(A.shape[0])
+
+
 def _polfed_params(ctx, s, smin, smax, sigma, emin, emax, tol, overextract, refine, window_type,
                    jackson, mmax, rtol, reorth_passes, extra_keep, seed, chunk, show_progress,
                    precision, refine_guard, refine_degree, refine_maxiter, refine_tol,
@@ -446,18 +467,18 @@ def polfed(A, nev: int, degree: int, s: int = 8, smin=None, smax=None, sigma: fl
     ``return_stats``) with the reference's return order rule
     (``polfed.jl:1148-1157``).
     """
+    n = _n_of(A)
+    q = None
+    if Q1 is not None:
+        q = np.asfortranarray(Q1, dtype=np.float64)
+        if q.shape != (n, s):
+            raise ValueError(f"Q1 must be {n}x{s}")
     ctx = ctx or default_context()
     dm, mf = _as_operator(A, ctx)
     p = _polfed_params(ctx, s, smin, smax, sigma, emin, emax, tol, overextract, refine,
                        window_type, jackson, mmax, rtol, reorth_passes, extra_keep, seed, chunk,
                        show_progress, precision, refine_guard, refine_degree, refine_maxiter,
                        refine_tol, refine_guard_degree)
-    n = mf.n if mf is not None else dm.n
-    q = None
-    if Q1 is not None:
-        q = np.asfortranarray(Q1, dtype=np.float64)
-        if q.shape != (n, s):
-            raise ValueError(f"Q1 must be {n}x{s}")
     h = C.c_void_p()
     if mf is not None:
         rc = ctx.lib.smalgo_polfed(ctx.h, None, mf.cfunc(), None, n, nev, degree, C.byref(p),
@@ -633,6 +654,7 @@ def chebyshev_poly_real_symm_sparse(A, emin: float, emax: float, smin=None, smax
                                     return_stats: bool = False, ctx: Optional[Context] = None):
     """Sparse form (``chebyshev_subspace_iteration.jl:667-757``): always runs
     ``maxiter`` iterations; returns ``(λsel, Xsel, ressel)`` sorted by λ."""
+    q0 = _start_block(Q0, _n_of(A), m0)
     ctx = ctx or default_context()
     dm, mf = _as_operator(A, ctx)
     if mf is not None:
@@ -649,7 +671,7 @@ def chebyshev_poly_real_symm_sparse(A, emin: float, emax: float, smin=None, smax
                      two_hit, seed, window_type)
     q = None
     if Q0 is not None:
-        q = np.asfortranarray(Q0[:, :min(m0, dm.n)], dtype=np.float64)
+        q = q0
     h = C.c_void_p()
     ctx.check(ctx.lib.smalgo_chebyshev(ctx.h, dm.h, L.MATBLOCK_FN(), None, dm.n, emin, emax,
                                        C.byref(p), _dptr(q) if q is not None else None,
@@ -673,6 +695,7 @@ def chebyshev_poly_real_symm_sparse_mf(matvec: Callable, n: int, emin: float, em
                                        ctx: Optional[Context] = None):
     """``matvec!`` form (``chebyshev_subspace_iteration.jl:771-848``): bounds are
     required; ``two_hit`` early exit."""
+    q0 = _start_block(Q0, n, m0)
     ctx = ctx or default_context()
     if smin is None or smax is None:
         raise ValueError("matrix-free form requires smin and smax")
@@ -681,7 +704,7 @@ def chebyshev_poly_real_symm_sparse_mf(matvec: Callable, n: int, emin: float, em
                      two_hit, seed, window_type)
     q = None
     if Q0 is not None:
-        q = np.asfortranarray(Q0[:, :min(m0, n)], dtype=np.float64)
+        q = q0
     h = C.c_void_p()
     rc = ctx.lib.smalgo_chebyshev(ctx.h, None, op.cfunc(), None, n, emin, emax, C.byref(p),
                                   _dptr(q) if q is not None else None, C.byref(h))

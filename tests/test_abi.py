@@ -116,3 +116,24 @@ def test_default_params_through_abi():
     lib.smalgo_cheb_default_params(C.byref(q))
     assert (q.m0, q.degree, q.maxiter, q.tol, q.res_gate) == (200, 80, 8, 1e-10, 1e-6)
     assert (q.jackson, q.debug, q.two_hit, q.seed, q.window_type) == (1, 0, 1, 1, 0)
+
+
+def test_wrappers_reject_malformed_start_blocks_before_the_device():
+    """Start blocks and projection bases of the wrong shape raise before any
+    context or device call (the library would read n·min(m0, n) values)."""
+    import scipy.sparse as sps
+
+    import sparse_matrix_algorithms_amd as sm
+    A = sps.identity(20, format="csr")
+    with pytest.raises(ValueError, match="Q0 must be 20"):
+        sm.chebyshev_poly_real_symm_sparse(A, -0.5, 0.5, m0=8, Q0=np.zeros((20, 5)))
+    with pytest.raises(ValueError, match="Q0 must be 20"):
+        sm.chebyshev_poly_real_symm_sparse(A, -0.5, 0.5, m0=8, Q0=np.zeros((19, 8)))
+    with pytest.raises(ValueError, match="Q0 must be 20"):
+        sm.chebyshev_poly_real_symm_sparse_mf(lambda y, x: None, 20, -0.5, 0.5, smin=-1, smax=1,
+                                              m0=8, Q0=np.zeros(20))
+    with pytest.raises(ValueError, match="Q1 must be 20x8"):
+        sm.polfed(A, 4, 10, s=8, Q1=np.zeros((20, 4)))
+    with pytest.raises(ValueError, match="as many rows"):
+        sm.orth_against(np.zeros((20, 3)), np.zeros((19, 2)))
+

@@ -119,6 +119,8 @@ def test_csr_upload_rejects_malformed_input(ctx):
     cases = [
         (3, [0, 1, 2, 3], [0, 1, 3], [1.0, 1.0, 1.0], 0, "column index out of range"),
         (3, [0, 2, 1, 3], [0, 1, 2], [1.0, 1.0, 1.0], 0, "nondecreasing"),
+        # row 0 claims 5 entries of a 2-entry matrix: rejected before idx/val are read
+        (2, [0, 5, 2], [0, 1], [1.0, 1.0], 0, "nondecreasing"),
         (3, [0, 1, 2, 3], [0, 1, 2], [1.0, 1.0, 1.0], 2, "index_base"),
         (3, [1, 2, 3, 4], [0, 2, 3], [1.0, 1.0, 1.0], 1, "column index out of range"),
         (-1, [0], [0], [0.0], 0, "n out of range"),
