@@ -21,6 +21,8 @@ template <> struct VecT<float, 2> { typedef float type __attribute__((ext_vector
 template <> struct VecT<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
 template <> struct VecT<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
 template <> struct VecT<double, 4> { typedef double type __attribute__((ext_vector_type(4))); };
+template <> struct VecT<float, 8> { typedef float type __attribute__((ext_vector_type(8))); };
+template <> struct VecT<double, 8> { typedef double type __attribute__((ext_vector_type(8))); };
 
 template <typename T, int V>
 __device__ __forceinline__ void ldv(const T* __restrict__ p, T (&r)[V]) {
@@ -246,9 +248,9 @@ static void clenshaw_dispatch(hipStream_t st, const DevCSR& A, int b, const T* Q
                                                     out, ldo, cq, a1, b1, s2, pcq, panels);
 #define SMALGO_CL_V(VV, RR) \
   SMALGO_CL(VV, 4, RR) SMALGO_CL(VV, 8, RR) SMALGO_CL(VV, 16, RR) SMALGO_CL(VV, 32, RR) SMALGO_CL(VV, 64, RR)
-  SMALGO_CL_V(1, 1) SMALGO_CL_V(2, 1) SMALGO_CL_V(4, 1)
-  SMALGO_CL_V(1, 2) SMALGO_CL_V(2, 2) SMALGO_CL_V(4, 2)
-  SMALGO_CL_V(1, 4) SMALGO_CL_V(2, 4) SMALGO_CL_V(4, 4)
+  SMALGO_CL_V(1, 1) SMALGO_CL_V(2, 1) SMALGO_CL_V(4, 1) SMALGO_CL_V(8, 1)
+  SMALGO_CL_V(1, 2) SMALGO_CL_V(2, 2) SMALGO_CL_V(4, 2) SMALGO_CL_V(8, 2)
+  SMALGO_CL_V(1, 4) SMALGO_CL_V(2, 4) SMALGO_CL_V(4, 4) SMALGO_CL_V(8, 4)
 #undef SMALGO_CL_V
 #undef SMALGO_CL
   throw Error(-3, "unsupported filter tile vec=" + std::to_string(t.vec) +
@@ -951,16 +953,34 @@ void zero_lower_cm(hipStream_t st, int b, double* C, int ldc) {
 }
 
 // plain streaming copy (16-byte vectors, grid-stride) for bandwidth calibration
+// Each workgroup copies one contiguous 16 KiB chunk per trip: every lane
+// issues its four 16-byte loads (256 lanes apart, so each is coalesced)
+// before its four stores, keeping four loads per lane in flight.
 __global__ __launch_bounds__(256) void stream_copy_kernel(const float4* __restrict__ a,
                                                           float4* __restrict__ b, size_t n4) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
-       i += (size_t)gridDim.x * blockDim.x)
-    b[i] = a[i];
+  constexpr int U = 4;
+  const size_t per = (size_t)U * 256;
+  for (size_t base = (size_t)blockIdx.x * per; base < n4; base += (size_t)gridDim.x * per) {
+    float4 v[U];
+    if (base + per <= n4) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = a[base + (size_t)u * 256 + threadIdx.x];
+#pragma unroll
+      for (int u = 0; u < U; ++u) b[base + (size_t)u * 256 + threadIdx.x] = v[u];
+    } else {
+      for (int u = 0; u < U; ++u) {
+        const size_t i = base + (size_t)u * 256 + threadIdx.x;
+        if (i < n4) b[i] = a[i];
+      }
+    }
+  }
 }
 void stream_copy(hipStream_t st, const void* src, void* dst, size_t bytes) {
   const size_t n4 = bytes / 16;
   if (!n4) return;
-  const int grid = 8 * device_cu_count();
+  // one trip per workgroup up to 32 workgroups per CU, grid-stride beyond
+  const size_t chunks = (n4 + 1023) / 1024;
+  const int grid = (int)std::min<size_t>(chunks, (size_t)32 * device_cu_count());
   hipLaunchKernelGGL(stream_copy_kernel, dim3(grid), dim3(256), 0, st,
                      reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n4);
   SMALGO_HIP(hipGetLastError());

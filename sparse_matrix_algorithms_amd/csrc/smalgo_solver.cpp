@@ -272,11 +272,11 @@ static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ld
     return 1;
   };
   const long long key = ((long long)sizeof(T) << 56) ^ ((long long)b << 32) ^
-                        ((long long)op.mat_id << 4) ^ (al(4) << 2) ^ (al(2) << 1);
+                        ((long long)op.mat_id << 4) ^ (al(8) << 3) ^ (al(4) << 2) ^ (al(2) << 1);
   auto it = cx.tuned.find(key);
   if (it != cx.tuned.end()) return it->second;
   std::vector<FilterTile> cand;
-  for (int v : {1, 2, 4}) {
+  for (int v : {1, 2, 4, 8}) {
     if (!al(v) || (v > 1 && b < v)) continue;
     for (int l : {4, 8, 16, 32, 64}) {
       const int w = l * v;  // chunk width in elements
@@ -1150,7 +1150,7 @@ struct SliceFilter {
       const FilterTile d = choose_tile(sizeof(T), w, {P0.p, P1.p}, {w});
       cand.push_back(FilterTile{d.vec, d.lpr, 1});
     } else {
-      for (int v : {1, 2, 4})
+      for (int v : {1, 2, 4, 8})
         for (int l : {4, 8, 16, 32, 64})
           if (w % v == 0 && l * v <= std::max(w, 4) * 2 && l * v * (int)sizeof(T) <= 2048)
             for (int rt : {1, 2, 4}) cand.push_back(FilterTile{v, l, rt});

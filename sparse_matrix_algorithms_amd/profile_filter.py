@@ -6,6 +6,8 @@ copy kernel of known size (bytes read = bytes written = ``--copy-mib`` MiB)
 so that FETCH_SIZE / WRITE_SIZE counters can be scaled to bytes.
 
     python -m sparse_matrix_algorithms_amd.profile_filter --L 48 --b 64 --prec fp32 --degree 50
+    python -m sparse_matrix_algorithms_amd.profile_filter --lattice xxz --L 20 --seed 20 --b 128
+    python -m sparse_matrix_algorithms_amd.profile_filter --copy-sweep
 
 With ``--slices N`` it runs the packed multi-slice filter instead: N slices of
 widths 8, 64, 16, 32, ... (the batched-slices benchmark's widths, cycled),
@@ -38,17 +40,34 @@ def main():
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--slices", type=int, default=0)
+    ap.add_argument("--lattice", choices=["anderson", "xxz"], default="anderson",
+                    help="xxz: random-field XXZ chain, Sz=0 sector, length --L (h=5, seed --seed)")
+    ap.add_argument("--copy-sweep", action="store_true",
+                    help="only time the copy kernel over buffers of 16 MiB to 4 GiB")
     args = ap.parse_args()
 
     import sparse_matrix_algorithms_amd as sm
-    from sparse_matrix_algorithms_amd.lattices import anderson3d, gaussian_block
+    from sparse_matrix_algorithms_amd.lattices import anderson3d, gaussian_block, xxz_sz0
 
-    A = anderson3d(args.L, args.W, args.seed)
-    n = A.shape[0]
     ctx = sm.default_context()
+    if args.copy_sweep:
+        sec = C.c_double()
+        for mib in (16, 64, 128, 256, 512, 1024, 4096):
+            nbytes = mib << 20
+            reps = max(3, min(50, (8 << 30) // nbytes))
+            ctx.check(ctx.lib.smalgo_copy_bench(ctx.h, nbytes, reps, C.byref(sec)))
+            print(f"copy {mib} MiB (read {mib} MiB + write {mib} MiB): {sec.value * 1e6:.1f} us "
+                  f"-> {2 * nbytes / sec.value / 1e9:.0f} GB/s", flush=True)
+        return 0
+    if args.lattice == "xxz":
+        A = xxz_sz0(args.L, 5.0, args.seed)
+    else:
+        A = anderson3d(args.L, args.W, args.seed)
+    n = A.shape[0]
     dm = sm.DeviceMatrix(A, ctx)
     smin, smax = dm.gershgorin_bounds()
-    co = sm.build_coeffs(args.degree, smin, smax, 0.0, -0.04, 0.04, "step")
+    e0 = float(A.diagonal().sum() / n)
+    co = sm.build_coeffs(args.degree, smin, smax, e0, e0 - 0.04, e0 + 0.04, "step")
     if args.slices:
         widths = [SLICE_WIDTHS[i % len(SLICE_WIDTHS)] for i in range(args.slices)]
         blocks = [gaussian_block(1 + i, n, w) for i, w in enumerate(widths)]
