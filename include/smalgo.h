@@ -108,6 +108,18 @@ int smalgo_build_coeffs(int degree, double smin, double smax, double sigma, doub
 int smalgo_apply_poly_block(smalgo_ctx* ctx, const smalgo_mat* A, int64_t b, const double* Q,
                             int64_t ldq, const double* coeffs, int K, double smin, double smax,
                             int precision, double* Y, int64_t ldy);
+/* No counterpart in the reference.  The filter of smalgo_apply_poly_block
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): it calls no
+ * allocator.  dQ and dY are DEVICE arrays, FP64, row-major n x b (element
+ * (i,c) at i*ld + c, ld >= b), not overlapping; coeffs[0..K] is a host
+ * array.  The recurrence's second block lives in a fixed per-device scratch
+ * of 64 MiB: n*ldy*8 bytes above that is SMALGO_ERR_ARG.  Calls that overlap
+ * on one GPU share that scratch and race.  A sparse matrix only; returns
+ * after the filter has finished on the context stream. */
+int smalgo_apply_poly_block_noalloc(smalgo_ctx* ctx, const smalgo_mat* A, int64_t b,
+                                    const double* dQ, int64_t ldq, const double* coeffs, int K,
+                                    double smin, double smax, double* dY, int64_t ldy);
 /* Same with a host operator (matrix-free form, polfed.jl:508-545). */
 int smalgo_apply_poly_block_mf(smalgo_ctx* ctx, int64_t n, smalgo_matblock_fn fn, void* user,
                                int64_t b, const double* Q, int64_t ldq, const double* coeffs,

@@ -80,6 +80,8 @@ SIGNATURES = [
     ("smalgo_apply_poly_block", C.c_int,
      [_P, _P, C.c_int64, _DP, C.c_int64, _DP, C.c_int, C.c_double, C.c_double, C.c_int, _DP,
       C.c_int64]),
+    ("smalgo_apply_poly_block_noalloc", C.c_int,
+     [_P, _P, C.c_int64, _P, C.c_int64, _DP, C.c_int, C.c_double, C.c_double, _P, C.c_int64]),
     ("smalgo_apply_poly_block_mf", C.c_int,
      [_P, C.c_int64, MATBLOCK_FN, _P, C.c_int64, _DP, C.c_int64, _DP, C.c_int, C.c_double,
       C.c_double, _DP, C.c_int64]),

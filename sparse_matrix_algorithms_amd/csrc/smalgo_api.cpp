@@ -94,6 +94,9 @@ int smalgo_ctx_create(int device, smalgo_ctx** out) {
     delete c;
     return SMALGO_ERR_INTERNAL;
   }
+  // device address of the no-allocation entry point's scratch, once per
+  // device (a failure is reported by that entry point, not here)
+  noalloc_scratch_init(device);
   *out = c;
   return SMALGO_OK;
 }
@@ -301,6 +304,40 @@ static int apply_poly_common(smalgo_ctx* ctx, const Op& op, int64_t b, const dou
     for (int j = 0; j < bb; ++j)
       SMALGO_HIP(hipMemcpyAsync(Y + (size_t)j * ldy, cm.p + (size_t)j * n, sizeof(double) * n,
                                 hipMemcpyDeviceToHost, ctx->stream));
+    SMALGO_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int smalgo_apply_poly_block_noalloc(smalgo_ctx* ctx, const smalgo_mat* A, int64_t b,
+                                    const double* dQ, int64_t ldq, const double* coeffs, int K,
+                                    double smin, double smax, double* dY, int64_t ldy) {
+  if (!ctx) return SMALGO_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (!A) throw Error(-1, "A is null");
+    if (b < 0 || K < 0 || !coeffs || (b > 0 && (!dQ || !dY))) throw Error(-1, "bad arguments");
+    if (ldq < b || ldy < b) throw Error(-1, "leading dimension smaller than b (row-major blocks)");
+    if (!(smax > smin)) throw Error(-1, "need smin < smax");
+    if (b == 0 || A->A.n == 0) return;
+    if (b > std::numeric_limits<int>::max() || ldq > std::numeric_limits<int>::max() ||
+        ldy > std::numeric_limits<int>::max())
+      throw Error(-1, "block width or leading dimension out of range");
+    const int64_t n = A->A.n;
+    size_t cap = 0;
+    double* scr = noalloc_scratch(ctx->device, &cap);
+    const double need = (double)n * (double)ldy * sizeof(double);
+    if (need > (double)cap)
+      throw Error(-1, "the no-allocation filter needs n*ldy*8 = " + std::to_string((long long)need) +
+                          " bytes of scratch; its device scratch holds " + std::to_string(cap) +
+                          " bytes (use smalgo_apply_poly_block)");
+    const char* q0 = reinterpret_cast<const char*>(dQ);
+    const char* y0 = reinterpret_cast<const char*>(dY);
+    if (q0 < y0 + (size_t)n * ldy * sizeof(double) && y0 < q0 + (size_t)n * ldq * sizeof(double))
+      throw Error(-1, "Q and Y overlap");
+    Op op;
+    op.A = &A->A;
+    op.n = (int)n;
+    op.mat_id = A->id;
+    apply_filter_noalloc(*ctx, op, (int)b, dQ, (int)ldq, coeffs, K, smin, smax, dY, (int)ldy, scr);
     SMALGO_HIP(hipStreamSynchronize(ctx->stream));
   });
 }

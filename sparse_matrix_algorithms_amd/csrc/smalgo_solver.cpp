@@ -260,19 +260,31 @@ static void filter_step(Ctx& cx, const Op& op, int b, const T* Q, int ldq, doubl
 // row) tile on the real operands and keep the fastest.  Rows per wavefront is
 // 64/lpr; the column chunk per row is lpr*vec elements.
 template <typename T>
+static int tile_alignment_ok(int v, const T* Q, int ldq, const T* buf0, const T* buf1, int ldb,
+                             const T* Y, int ldy) {
+  for (const void* p : {(const void*)Q, (const void*)buf0, (const void*)buf1, (const void*)Y})
+    if (reinterpret_cast<uintptr_t>(p) % (sizeof(T) * v)) return 0;
+  for (int ld : {ldq, ldb, ldy})
+    if (ld % v) return 0;
+  return 1;
+}
+
+// key of the autotuner's cache: precision, width, matrix and operand alignment
+template <typename T>
+static long long tile_key(const Op& op, int b, const T* Q, int ldq, const T* buf0, const T* buf1,
+                          int ldb, const T* Y, int ldy) {
+  auto al = [&](int v) { return tile_alignment_ok<T>(v, Q, ldq, buf0, buf1, ldb, Y, ldy); };
+  return ((long long)sizeof(T) << 56) ^ ((long long)b << 32) ^ ((long long)op.mat_id << 4) ^
+         (al(8) << 3) ^ (al(4) << 2) ^ (al(2) << 1);
+}
+
+template <typename T>
 static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ldq, T* buf0, T* buf1,
                                 int ldb, T* Y, int ldy) {
   const FilterTile def = choose_tile(sizeof(T), b, {Q, buf0, buf1, Y}, {ldq, ldb, ldy});
   if (!cx.autotune) return def;
-  auto al = [&](int v) {
-    for (const void* p : {(const void*)Q, (const void*)buf0, (const void*)buf1, (const void*)Y})
-      if (reinterpret_cast<uintptr_t>(p) % (sizeof(T) * v)) return 0;
-    for (int ld : {ldq, ldb, ldy})
-      if (ld % v) return 0;
-    return 1;
-  };
-  const long long key = ((long long)sizeof(T) << 56) ^ ((long long)b << 32) ^
-                        ((long long)op.mat_id << 4) ^ (al(8) << 3) ^ (al(4) << 2) ^ (al(2) << 1);
+  auto al = [&](int v) { return tile_alignment_ok<T>(v, Q, ldq, buf0, buf1, ldb, Y, ldy); };
+  const long long key = tile_key<T>(op, b, Q, ldq, buf0, buf1, ldb, Y, ldy);
   auto it = cx.tuned.find(key);
   if (it != cx.tuned.end()) return it->second;
   std::vector<FilterTile> cand;
@@ -313,7 +325,7 @@ static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ld
 // out = cq*Q + a1*(A X1) + b1*X1 - s2*X2 with X1/X2 given as (pointer,
 // implicit multiple of Q).
 template <typename T, typename StepFn>
-static void clenshaw_recurrence(const std::vector<double>& co, double smin, double smax, T* buf0,
+static void clenshaw_recurrence(const double* co, int K, double smin, double smax, T* buf0,
                                 T* buf1, T* Y, StepFn&& step) {
   // Block Clenshaw exactly as apply_poly_block! (polfed.jl:464-501):
   //   B_{K+1} = B_{K+2} = 0
@@ -323,7 +335,6 @@ static void clenshaw_recurrence(const std::vector<double>& co, double smin, doub
   // steps have B1 = c_K Q and B2 ∈ {0, c_K Q}; they are folded into the
   // Q term instead of materialised, so the zero SpMM of the reference's first
   // step is never launched.  From the third step on TMP overwrites B2 in place.
-  const int K = (int)co.size() - 1;
   const double c = 0.5 * (smin + smax);
   const double invd = 2.0 / (smax - smin);
   const double al = 2.0 * invd, be = -2.0 * c * invd;
@@ -423,7 +434,8 @@ void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::
         launch_clenshaw_step<T>(cx.stream, *op.A, bp, Qn, pw, (T)cqe, x1p, pw, (T)a1e, (T)b1e,
                                 x2p, pw, (T)s2e, out, pw, tile);
       };
-      clenshaw_recurrence<T>(co, smin, smax, P0.p, P1.p, Yp.p + (size_t)pn * ps, step);
+      clenshaw_recurrence<T>(co.data(), (int)co.size() - 1, smin, smax, P0.p, P1.p,
+                             Yp.p + (size_t)pn * ps, step);
     }
     from_panels<T>(cx.stream, n, b, pw, Yp.p, Y, ldy);
     return;
@@ -441,7 +453,7 @@ void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::
     const int ldo = (out == Y) ? ldy : ldb;
     filter_step<T>(cx, op, b, Q, ldq, cq, a1, b1, X1, X2, ldb, out, ldo, pbuf, tp);
   };
-  clenshaw_recurrence<T>(co, smin, smax, buf0, buf1, Y, step);
+  clenshaw_recurrence<T>(co.data(), (int)co.size() - 1, smin, smax, buf0, buf1, Y, step);
 }
 template void apply_filter<double>(Ctx&, const Op&, int, const double*, int,
                                    const std::vector<double>&, double, double, double*, int,
@@ -449,6 +461,44 @@ template void apply_filter<double>(Ctx&, const Op&, int, const double*, int,
 template void apply_filter<float>(Ctx&, const Op&, int, const float*, int,
                                   const std::vector<double>&, double, double, float*, int, float*,
                                   float*, int, double*);
+
+// Y = p(Â) Q for a CSR operator with no allocation of any kind: the
+// ping-pong pair of the recurrence is (Y, scratch), ordered so that the last
+// step's B_2 is Y itself (the kernel allows out to alias X2 element by
+// element) and never its gathered B_1; Y and scratch share the leading
+// dimension ldy.  The tile is the autotuner's cached choice for this shape if
+// one exists, else the default tile: nothing is timed.  Row-major launches
+// only (no panel repacking).
+void apply_filter_noalloc(Ctx& cx, const Op& op, int b, const double* Q, int ldq, const double* co,
+                          int K, double smin, double smax, double* Y, int ldy, double* scratch) {
+  if (op.is_callback()) throw Error(-1, "the no-allocation filter needs a sparse matrix");
+  // which buffer of the pair is the final B_2 (and which the final B_1)?
+  double* const L0 = reinterpret_cast<double*>(uintptr_t(16));
+  double* const L1 = reinterpret_cast<double*>(uintptr_t(32));
+  double* const LY = reinterpret_cast<double*>(uintptr_t(48));
+  const double* last1 = nullptr;
+  const double* last2 = nullptr;
+  clenshaw_recurrence<double>(co, K, smin, smax, L0, L1, LY,
+                              [&](double, double, double, FBlk<double> X1, FBlk<double> X2,
+                                  double* out) {
+                                if (out == LY) {
+                                  last1 = X1.p;
+                                  last2 = X2.p;
+                                }
+                              });
+  const bool y_first = last2 ? last2 == L0 : last1 != L0;
+  double* buf0 = y_first ? Y : scratch;
+  double* buf1 = y_first ? scratch : Y;
+  FilterTile tile = choose_tile(sizeof(double), b, {Q, buf0, buf1, Y}, {ldq, ldy, ldy});
+  auto it = cx.tuned.find(tile_key<double>(op, b, Q, ldq, buf0, buf1, ldy, Y, ldy));
+  if (it != cx.tuned.end()) tile = it->second;
+  cx.last_tile = tile;
+  cx.last_launches_per_step = 1;
+  auto step = [&](double cq, double a1, double b1, FBlk<double> X1, FBlk<double> X2, double* out) {
+    filter_step<double>(cx, op, b, Q, ldq, cq, a1, b1, X1, X2, ldy, out, ldy, nullptr, &tile);
+  };
+  clenshaw_recurrence<double>(co, K, smin, smax, buf0, buf1, Y, step);
+}
 
 // ---------------------------------------------------------------------------
 // CholQR2 with a shifted-CholQR3 fallback (Fukaya et al., SIAM J. Sci.

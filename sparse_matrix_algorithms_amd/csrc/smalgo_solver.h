@@ -138,6 +138,19 @@ void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::
                   double smin, double smax, T* Y, int ldy, T* buf0, T* buf1, int ldb,
                   double* pbuf /* callback staging, n x ldb doubles, may be null for CSR */);
 
+// The same filter for a CSR operator without allocating: Y (row-major, ld
+// ldy) and `scratch` (n x ldy doubles) are the ping-pong pair; coefficients
+// co[0..K] on the host.
+void apply_filter_noalloc(Ctx& cx, const Op& op, int b, const double* Q, int ldq, const double* co,
+                          int K, double smin, double smax, double* Y, int ldy, double* scratch);
+
+// Device scratch of the no-allocation entry point (smalgo_noalloc.hip): a
+// static device array whose address is looked up once per device at context
+// creation.  noalloc_scratch returns the calling device's address and its
+// size in bytes, or throws if the lookup failed.
+void noalloc_scratch_init(int device);
+double* noalloc_scratch(int device, size_t* bytes);
+
 // Y = A X (CSR or callback)
 template <typename T>
 void apply_A(Ctx& cx, const Op& op, int b, const T* X, int ldx, T* Y, int ldy);
