@@ -239,7 +239,7 @@ def main():
     bytes_per_launch = st["spmm_bytes"] / max(launches, 1)
     achieved = bytes_per_launch / avg_launch / 1e9
     tname = "float" if head_prec == "fp32" else "double"
-    kname = (f"clenshaw_kernel<{tname},{int(st['tile_vec'])},{int(st['tile_lpr'])},"
+    kname = (f"{kernel_base(st)}<{tname},{int(st['tile_vec'])},{int(st['tile_lpr'])},"
              f"{int(st['tile_rt'])},false>")
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs of
     # the same kernel and tile on this workload (scaled by the copy-kernel calibration)
@@ -268,19 +268,28 @@ def main():
             l2, _, r2, s2, d2 = solve("fp64", KL64, mmaxL64)
             detail["fp64_" + src], set2 = summary(l2, r2, s2, d2, KL64, mmaxL64, "fp64")
             detail["fp64_" + src]["same_set_as_fp64_plain"] = same(set2, set64)
+            # the precision change alone: the headline against FP64 with the same
+            # parameter rule (the 0.7/1.08 mixed-path scaling of that rule included)
+            detail["speedup_vs_fp64_same_params"] = round(
+                value / max(detail["fp64_" + src]["eigenpairs_per_s"], 1e-12), 3)
             log(f"[bench] fp64 {src}: {d2:.3f}s {s2['line']}")
             Km, mmaxm = kpm.plain_parameters(n, args.nev, emin, emax, smin, smax, args.s,
                                              args.kappa, args.mmax_factor, precision="fp32")
             l3, _, r3, s3, d3 = solve("fp32", Km, mmaxm)
             detail["mixed_plain"], set3 = summary(l3, r3, s3, d3, Km, mmaxm, "fp32")
             detail["mixed_plain"]["same_set_as_fp64_plain"] = same(set3, set64)
+            # mixed precision with the plain rule against FP64 with the plain rule
+            detail["mixed_plain_vs_fp64_plain"] = round(
+                detail["mixed_plain"]["eigenpairs_per_s"] /
+                max(detail["fp64_plain"]["eigenpairs_per_s"], 1e-12), 3)
             log(f"[bench] mixed plain: {d3:.3f}s {s3['line']}")
 
     cpu = None
     if not args.skip_cpu:
         steps64 = st64_steps if "fp64_plain" in detail else st["filter_steps"]
+        blocks64 = st64["block_steps"] if "fp64_plain" in detail else st["block_steps"]
         cpu = cpu_baseline(A, args.s, smin, smax, emin, emax, steps64, nvalid, args.cpu_seconds,
-                           "of the FP64 solve")
+                           "of the FP64 plain solve", blocks=blocks64, nev=args.nev)
 
     out = {
         "metric": "eigenpairs/s at rel. resid <=1e-10, POLFED FP32+refine vs FP64; SpMM HBM GB/s",
@@ -308,15 +317,61 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(A, b, smin, smax, emin, emax, steps, pairs, seconds, what):
-    """Time the oracle's OpenMP FP64 Clenshaw filter (the reference algorithm's
-    dominant cost) on a bounded sample at block width b and project it over
-    `steps` fused filter steps: eigenpairs/s of a filter-only CPU solve."""
+def kernel_base(st):
+    """Name of the fused filter kernel the autotuner picked (the CSR tile staged
+    in LDS or read per lane)."""
+    return "clenshaw_lds_kernel" if st.get("tile_lds", 0) else "clenshaw_kernel"
+
+
+def cpu_info():
+    """(model name, logical CPUs) of the host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
+
+
+def cpu_threads():
+    """Host threads for the CPU leg: the job's CPU share as OMP_NUM_THREADS
+    states it (a one-GPU job on the benchmark hosts gets 16), else every
+    logical CPU, as the reference does with Sys.CPU_THREADS (polfed.jl:1241)."""
+    try:
+        t = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        t = 0
+    return t if t > 0 else (os.cpu_count() or 1)
+
+
+def cpu_baseline(A, b, smin, smax, emin, emax, steps, pairs, seconds, what, blocks=None,
+                 nev=None):
+    """The oracle's FP64 restatement of the reference on the host cores, timed
+    on a bounded sample and projected over the GPU solve's shape:
+
+    * filter: the OpenMP CSR Clenshaw filter at block width b, timed for about
+      `seconds`, times `steps` fused steps;
+    * orthogonalisation (when `blocks` is given): one block step of the
+      basis (polfed.jl:1066-1097: W -= Q_{j-1}B_{j-1}ᵀ, A_j = Q_jᵀW,
+      W -= Q_jA_j, reorth_full! against j blocks, block_qr_factor) at the mean
+      history length, times `blocks`;
+    * extraction (when `nev` is given): eigen(T) of the blocks·b reduced
+      matrix, the Ritz lift and Rayleigh quotients of min(blocks·b, 2nev)
+      candidates, and refine_original_rr of nev+8 vectors (polfed.jl:892-984),
+      timed once.
+
+    Returns eigenpairs/s of the projected CPU solve."""
+    import numpy as np
     import sparse_matrix_algorithms_amd as sm
     from oracle import clenshaw_c
+    from oracle import polfed_ref as P
     from sparse_matrix_algorithms_amd.lattices import gaussian_block
     n = A.shape[0]
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
+    model, ncpu = cpu_info()
     Q = gaussian_block(1, n, b)
     co = sm.build_coeffs(16, smin, smax, 0.5 * (emin + emax), emin, emax, "step")
     clenshaw_c.apply_poly_block(A, Q, co, smin, smax, nthreads=threads)  # warm
@@ -327,13 +382,50 @@ def cpu_baseline(A, b, smin, smax, emin, emax, steps, pairs, seconds, what):
         t_cpu += time.perf_counter() - tt
         k += len(co)
     per_step = t_cpu / k
-    t_solve = per_step * steps
-    log(f"[bench] cpu: {per_step * 1e3:.2f} ms/step x {int(steps)} -> {t_solve:.1f}s")
+    t_filter = per_step * steps
+    t_orth = t_extract = 0.0
+    parts = [f"filter {per_step * 1e3:.2f} ms/step x {int(steps)} steps = {t_filter:.1f}s"]
+    rng = np.random.default_rng(3)
+    if blocks:
+        j = max(1, int(round(blocks / 2)))
+        hist = [np.linalg.qr(rng.standard_normal((n, b)))[0] for _ in range(j)]
+        Qj, Qp = hist[-1], hist[-2] if j > 1 else hist[-1]
+        Bp = np.triu(rng.standard_normal((b, b)))
+        W = rng.standard_normal((n, b))
+        tt = time.perf_counter()
+        W -= Qp @ Bp.T
+        Aj = P.sym_upper(0.5 * (Qj.T @ W + W.T @ Qj))
+        W -= Qj @ Aj
+        P.reorth_full(W, hist, Qj, passes=1)
+        P.block_qr_factor(W)
+        t_step = time.perf_counter() - tt
+        t_orth = t_step * blocks
+        parts.append(f"orthogonalisation {t_step:.3f}s per block step (history {j} blocks) x "
+                     f"{int(blocks)} = {t_orth:.1f}s")
+        del hist, W
+    if nev and blocks:
+        M = int(blocks) * b
+        m = min(M, 2 * nev)
+        T = rng.standard_normal((M, M))
+        T = 0.5 * (T + T.T)
+        tt = time.perf_counter()
+        _, U = np.linalg.eigh(T)
+        for j0 in range(0, m, 64):   # the lift in 64-wide chunks, as polfed.jl:941-958
+            Vc = rng.standard_normal((n, M)) if j0 == 0 else Vc
+            X = Vc @ U[:, j0:min(j0 + 64, m)]
+            P.rayleigh(A, X)
+        P.refine_original_rr(A, rng.standard_normal((n, nev + 8)))
+        t_extract = time.perf_counter() - tt
+        parts.append(f"extraction (eigen of {M}x{M}, lift + Rayleigh of {m}, refine of "
+                     f"{nev + 8}) {t_extract:.1f}s")
+        del Vc, T, U
+    t_solve = t_filter + t_orth + t_extract
+    log(f"[bench] cpu: {'; '.join(parts)} -> {t_solve:.1f}s")
     return dict(value=round(pairs / t_solve, 4), unit="eigenpairs/s", cores=threads, kind="port",
-                sample=(f"oracle OpenMP FP64 CSR Clenshaw filter, n={n}, b={b}: {k} steps in "
-                        f"{t_cpu:.1f}s = {per_step * 1e3:.2f} ms/step, projected over the "
-                        f"{int(steps)} filter steps {what} (filter only, no orthogonalisation or "
-                        f"extraction: a lower bound on CPU time)"))
+                cpu_model=model, nproc=ncpu,
+                sample=(f"oracle FP64 restatement on {threads} host threads ({model}, {ncpu} "
+                        f"logical CPUs), n={n}, b={b}: " + "; ".join(parts) +
+                        f"; projected CPU solve {t_solve:.1f}s {what}"))
 
 
 def filter_roofline(st, prec, percol, bytes_key="spmm_bytes", launches=None):
@@ -345,7 +437,7 @@ def filter_roofline(st, prec, percol, bytes_key="spmm_bytes", launches=None):
     bpl = st[bytes_key] / max(launches, 1)
     achieved = bpl / avg_launch / 1e9
     tname = "float" if prec == "fp32" else "double"
-    kname = (f"clenshaw_kernel<{tname},{int(st['tile_vec'])},{int(st['tile_lpr'])},"
+    kname = (f"{kernel_base(st)}<{tname},{int(st['tile_vec'])},{int(st['tile_lpr'])},"
              f"{int(st['tile_rt'])},{'true' if percol else 'false'}>")
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=kname,
@@ -566,7 +658,7 @@ def bench_xxz(args):
     cpu = None
     if not args.skip_cpu:
         cpu = cpu_baseline(A, s, smin, smax, emin, emax, st["filter_steps"], nv, args.cpu_seconds,
-                           "of the solve")
+                           "of the solve", blocks=st["block_steps"], nev=nev)
     out = {
         "metric": "eigenpairs/s at rel. resid <=1e-10, POLFED on the XXZ chain",
         "value": round(nv / dt, 3), "unit": "eigenpairs/s", "n_gpus": 1, "steps": args.steps,

@@ -51,3 +51,49 @@ int oracle_clenshaw(int64_t n, int64_t b, const int64_t* ptr, const int64_t* idx
   }
   return 0;
 }
+
+/*
+ * The same filter on row-major n x b blocks (element (i,c) at i*b + c), with
+ * OpenMP over rows: each step reads the matrix once for all b columns, so a
+ * wide block costs one matrix pass per step instead of one per column.
+ */
+static void step_rm(int64_t n, int64_t b, const int64_t* ptr, const int64_t* idx,
+                    const double* val, const double* q, double ck, const double* b1, double al,
+                    double be, const double* b2, double* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    double* o = out + i * b;
+    const double* qi = q + i * b;
+    const double* x = b1 + i * b;
+    const double* y = b2 ? b2 + i * b : 0;
+    for (int64_t c = 0; c < b; ++c) o[c] = ck * qi[c] + be * x[c] - (y ? y[c] : 0.0);
+    for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p) {
+      const double w = al * val[p];
+      const double* xr = b1 + idx[p] * b;
+      for (int64_t c = 0; c < b; ++c) o[c] += w * xr[c];
+    }
+  }
+}
+
+int oracle_clenshaw_rm(int64_t n, int64_t b, const int64_t* ptr, const int64_t* idx,
+                       const double* val, const double* Q, const double* coeffs, int K,
+                       double smin, double smax, double* Y, double* work, int nthreads) {
+  const double c = 0.5 * (smin + smax);
+  const double invd = 2.0 / (smax - smin);
+  const double al = 2.0 * invd, be = -2.0 * c * invd;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  double* B1 = work;
+  double* B2 = work + (size_t)n * b;
+  double* T = work + (size_t)2 * n * b;
+  memset(B1, 0, sizeof(double) * n * b);
+  memset(B2, 0, sizeof(double) * n * b);
+  for (int k = K + 1; k >= 2; --k) {
+    step_rm(n, b, ptr, idx, val, Q, coeffs[k - 1], B1, al, be, B2, T);
+    double* t = B2;
+    B2 = B1;
+    B1 = T;
+    T = t;
+  }
+  step_rm(n, b, ptr, idx, val, Q, coeffs[0], B1, invd, -c * invd, B2, Y);
+  return 0;
+}

@@ -55,6 +55,7 @@ struct FilterTile {
   int vec = 1;   // elements per lane per load
   int lpr = 64;  // lanes per row (rows per wavefront = 64/lpr)
   int rt = 1;    // consecutive rows per lane team
+  int lds = 0;   // 1: the workgroup's CSR tile is staged in LDS first
 };
 
 template <typename T>
@@ -129,6 +130,12 @@ void rm_to_cm(hipStream_t st, int n, int k, const TS* S, int lds, const int* sel
 template <typename T>
 void gather_cols_rm(hipStream_t st, int n, int k, const T* S, int lds, const int* sel, T* D, int ldd);
 void add_diag(hipStream_t st, int b, double* C, int ldc, double v);
+// C = A·B for upper-triangular column-major b x b (ld b) matrices
+void tri_mul_upper(hipStream_t st, int b, const double* A, const double* B, double* C);
+// out[0..1] = info[0..1], out[2..2+b) = diag(R)  (R column-major, ld b)
+void pack_qr_status(hipStream_t st, int b, const int* info, const double* R, double* out);
+// s x s block, column-major: rows < rn and columns < r of R (ld r), zero elsewhere
+void b_block(hipStream_t st, int s, int r, int rn, const double* R, double* Bt);
 void zero_lower_cm(hipStream_t st, int b, double* C, int ldc);
 void launch_clenshaw_step_pre(hipStream_t st, int n, int b, const double* Q, int ldq, double cq,
                               const double* P, int ldp, const double* X1, int ld1, double a1,

@@ -215,10 +215,164 @@ __global__ __launch_bounds__(256) void clenshaw_kernel(
   }
 }
 
+// The same fused step with the workgroup's CSR tile staged in LDS first:
+// the rows' pointers, column indices and values of one row tile are loaded
+// once per workgroup with coalesced loads, so the LPR lanes of a team read
+// them from LDS (broadcast reads) instead of each issuing its own global
+// index and value loads per nonzero.  Tiles with more than 12 nonzeros per
+// row on average take the global path.  Gather form only (X1 set, P null).
+template <typename T, int V, int LPR, int RT, bool PERCOL>
+__global__ __launch_bounds__(256) void clenshaw_lds_kernel(
+    int n, int b, int nrowtiles, int ntiles, int per_xcd, const int* __restrict__ rowptr,
+    const int* __restrict__ colidx, const T* __restrict__ val, const T* __restrict__ Q, int ldq,
+    const T* __restrict__ X1, int ld1, const T* X2, int ld2, T* out, int ldo, T cq, T a1, T b1,
+    T s2, const T* __restrict__ pcq, long long chunk_stride) {
+  constexpr int RPW = 64 / LPR;
+  constexpr int RPB = 4 * RPW * RT;
+  constexpr int MAXE = RPB * 12;
+  __shared__ int s_ptr[RPB + 1];
+  __shared__ int s_col[MAXE];
+  __shared__ T s_val[MAXE];
+  const int tile = per_xcd ? xcd_tile(per_xcd) : (int)blockIdx.x;
+  if (tile >= ntiles) return;  // whole workgroup
+  const int chunk = tile / nrowtiles;
+  const int rtile = tile - chunk * nrowtiles;
+  const int rbase = rtile * RPB;
+  const int nrows = min(RPB, n - rbase);  // >= 1: rtile < nrowtiles
+  const int e0 = rowptr[rbase];
+  const int ne = rowptr[rbase + nrows] - e0;
+  const bool staged = ne <= MAXE;  // uniform over the workgroup
+  if (staged) {
+    for (int t = threadIdx.x; t <= nrows; t += 256) s_ptr[t] = rowptr[rbase + t] - e0;
+    for (int t = threadIdx.x; t < ne; t += 256) {
+      s_col[t] = colidx[e0 + t];
+      s_val[t] = val[e0 + t];
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int row0 = rbase + (wave * RPW + lane / LPR) * RT;
+  if (row0 >= n) return;
+  const int cic = (lane % LPR) * V;
+  const int c = chunk * (LPR * V) + cic;
+  if (c >= b) return;
+  const long long coff = chunk_stride ? (long long)chunk * chunk_stride + cic : c;
+  if (Q) Q += coff;
+  X1 += coff;
+  if (X2) X2 += coff;
+  out += coff;
+  const bool full = (c + V <= b);
+  int p0[RT], len[RT];
+  int maxlen = 0;
+#pragma unroll
+  for (int r = 0; r < RT; ++r) {
+    const int row = min(row0 + r, n - 1);
+    if (staged) {
+      p0[r] = s_ptr[row - rbase];
+      len[r] = row0 + r < n ? s_ptr[row - rbase + 1] - p0[r] : 0;
+    } else {
+      p0[r] = rowptr[row];
+      len[r] = row0 + r < n ? rowptr[row + 1] - p0[r] : 0;
+    }
+    maxlen = max(maxlen, len[r]);
+  }
+  auto colv = [&](int q, bool in, int& cidx, T& w) {
+    if (staged) {
+      cidx = s_col[q];
+      w = in ? s_val[q] : T(0);
+    } else {
+      cidx = colidx[q];
+      w = in ? val[q] : T(0);
+    }
+  };
+  T acc[RT][V];
+#pragma unroll
+  for (int r = 0; r < RT; ++r)
+#pragma unroll
+    for (int e = 0; e < V; ++e) acc[r][e] = T(0);
+  if (full) {
+    constexpr int U = RT >= 4 ? 1 : (RT == 2 ? 2 : 4);
+    for (int j = 0; j < maxlen; j += U) {
+      int col[U][RT];
+      T w[U][RT];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+          const int jj = j + u;
+          if (len[r] > 0) {
+            colv(p0[r] + min(jj, len[r] - 1), jj < len[r], col[u][r], w[u][r]);
+          } else {
+            col[u][r] = 0;
+            w[u][r] = T(0);
+          }
+        }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+          T x[V];
+          ldv<T, V>(X1 + (size_t)col[u][r] * ld1, x);
+#pragma unroll
+          for (int e = 0; e < V; ++e) acc[r][e] += w[u][r] * x[e];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+      const int row = row0 + r;
+      if (row >= n) break;
+      T res[V];
+#pragma unroll
+      for (int e = 0; e < V; ++e) res[e] = a1 * acc[r][e];
+      if (Q != nullptr) {
+        T q[V];
+        ldv<T, V>(Q + (size_t)row * ldq, q);
+#pragma unroll
+        for (int e = 0; e < V; ++e) res[e] = (PERCOL ? pcq[c + e] : cq) * q[e] + res[e];
+      }
+      {
+        T x[V];
+        ldv<T, V>(X1 + (size_t)row * ld1, x);
+#pragma unroll
+        for (int e = 0; e < V; ++e) res[e] += b1 * x[e];
+      }
+      if (X2 != nullptr) {
+        T y[V];
+        ldv_alias<T, V>(X2 + (size_t)row * ld2, y);
+#pragma unroll
+        for (int e = 0; e < V; ++e) res[e] -= s2 * y[e];
+      }
+      stv<T, V>(out + (size_t)row * ldo, res);
+    }
+  } else {
+    for (int r = 0; r < RT; ++r) {
+      const int row = row0 + r;
+      if (row >= n) break;
+      for (int e = 0; e < V && c + e < b; ++e) {
+        const int cc = c + e;
+        T a = T(0);
+        for (int q = p0[r]; q < p0[r] + len[r]; ++q) {
+          int ci;
+          T w;
+          colv(q, true, ci, w);
+          a += w * X1[(size_t)ci * ld1 + e];
+        }
+        T res = a1 * a;
+        if (Q != nullptr) res = (PERCOL ? pcq[cc] : cq) * Q[(size_t)row * ldq + e] + res;
+        res += b1 * X1[(size_t)row * ld1 + e];
+        if (X2 != nullptr) res -= s2 * X2[(size_t)row * ld2 + e];
+        out[(size_t)row * ldo + e] = res;
+      }
+    }
+  }
+}
+
 template <typename T, int V, int LPR, int RT, bool PERCOL>
 static void clenshaw_launch_t(hipStream_t st, const DevCSR& A, int b, const T* Q, int ldq,
                               const T* X1, int ld1, const T* P, int ldp, const T* X2, int ld2,
-                              T* out, int ldo, T cq, T a1, T b1, T s2, const T* pcq, bool panels) {
+                              T* out, int ldo, T cq, T a1, T b1, T s2, const T* pcq, bool panels,
+                              bool lds) {
   constexpr int RPB = 4 * (64 / LPR) * RT;
   const int nrowtiles = (A.n + RPB - 1) / RPB;
   const int nchunks = (b + LPR * V - 1) / (LPR * V);
@@ -230,10 +384,15 @@ static void clenshaw_launch_t(hipStream_t st, const DevCSR& A, int b, const T* Q
     per_xcd = 0;
     grid = (int)nt;
   }
-  hipLaunchKernelGGL((clenshaw_kernel<T, V, LPR, RT, PERCOL>), dim3(grid), dim3(256), 0, st, A.n, b,
-                     nrowtiles, (int)nt, per_xcd, A.rowptr, A.col, A.vals<T>(), Q, ldq, X1, ld1, P,
-                     ldp, X2, ld2, out, ldo, cq, a1, b1, s2, pcq,
-                     panels ? (long long)A.n * (LPR * V) : 0LL);
+  const long long cs = panels ? (long long)A.n * (LPR * V) : 0LL;
+  if (lds && X1 != nullptr && P == nullptr)
+    hipLaunchKernelGGL((clenshaw_lds_kernel<T, V, LPR, RT, PERCOL>), dim3(grid), dim3(256), 0, st,
+                       A.n, b, nrowtiles, (int)nt, per_xcd, A.rowptr, A.col, A.vals<T>(), Q, ldq,
+                       X1, ld1, X2, ld2, out, ldo, cq, a1, b1, s2, pcq, cs);
+  else
+    hipLaunchKernelGGL((clenshaw_kernel<T, V, LPR, RT, PERCOL>), dim3(grid), dim3(256), 0, st, A.n,
+                       b, nrowtiles, (int)nt, per_xcd, A.rowptr, A.col, A.vals<T>(), Q, ldq, X1,
+                       ld1, P, ldp, X2, ld2, out, ldo, cq, a1, b1, s2, pcq, cs);
   SMALGO_HIP(hipGetLastError());
 }
 
@@ -245,7 +404,8 @@ static void clenshaw_dispatch(hipStream_t st, const DevCSR& A, int b, const T* Q
 #define SMALGO_CL(VV, LL, RR)                                                                  \
   if (t.vec == VV && t.lpr == LL && t.rt == RR)                                                \
     return clenshaw_launch_t<T, VV, LL, RR, PERCOL>(st, A, b, Q, ldq, X1, ld1, P, ldp, X2, ld2, \
-                                                    out, ldo, cq, a1, b1, s2, pcq, panels);
+                                                    out, ldo, cq, a1, b1, s2, pcq, panels,  \
+                                                    t.lds != 0);
 #define SMALGO_CL_V(VV, RR) \
   SMALGO_CL(VV, 4, RR) SMALGO_CL(VV, 8, RR) SMALGO_CL(VV, 16, RR) SMALGO_CL(VV, 32, RR) SMALGO_CL(VV, 64, RR)
   SMALGO_CL_V(1, 1) SMALGO_CL_V(2, 1) SMALGO_CL_V(4, 1) SMALGO_CL_V(8, 1)
@@ -935,6 +1095,53 @@ __global__ void add_diag_kernel(int b, double* C, int ldc, double v) {
 void add_diag(hipStream_t st, int b, double* C, int ldc, double v) {
   if (b <= 0) return;
   hipLaunchKernelGGL(add_diag_kernel, dim3((b + 255) / 256), dim3(256), 0, st, b, C, ldc, v);
+  SMALGO_HIP(hipGetLastError());
+}
+
+// C = A·B for upper-triangular column-major b x b matrices (ld b): entry
+// (i, j), j >= i, is the sum over k in [i, j] of A(i,k)·B(k,j); C may not
+// alias A or B.
+__global__ void tri_mul_upper_kernel(int b, const double* __restrict__ A,
+                                     const double* __restrict__ B, double* __restrict__ C) {
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b || j >= b) return;
+  double acc = 0.0;
+  for (int k = i; k <= j; ++k) acc += A[(size_t)k * b + i] * B[(size_t)j * b + k];
+  C[(size_t)j * b + i] = acc;  // zero below the diagonal (empty sum)
+}
+void tri_mul_upper(hipStream_t st, int b, const double* A, const double* B, double* C) {
+  if (b <= 0) return;
+  dim3 blk(16, 16), grid((b + 15) / 16, (b + 15) / 16);
+  hipLaunchKernelGGL(tri_mul_upper_kernel, grid, blk, 0, st, b, A, B, C);
+  SMALGO_HIP(hipGetLastError());
+}
+
+// out = [info[0], info[1], R(0,0), R(1,1), ..., R(b-1,b-1)] (R column-major, ld b)
+__global__ void pack_qr_status_kernel(int b, const int* __restrict__ info,
+                                      const double* __restrict__ R, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2) out[i] = (double)info[i];
+  if (i < b) out[2 + i] = R[(size_t)i * b + i];
+}
+void pack_qr_status(hipStream_t st, int b, const int* info, const double* R, double* out) {
+  hipLaunchKernelGGL(pack_qr_status_kernel, dim3((b + 2 + 255) / 256), dim3(256), 0, st, b, info, R,
+                     out);
+  SMALGO_HIP(hipGetLastError());
+}
+
+// Bt(i, c) = R(i, c) for i < rn, else 0: the s x s device block (column-major,
+// ld s) that the next block step reads as B_jᵀ, from R (r x r, ld r)
+__global__ void b_block_kernel(int s, int r, int rn, const double* __restrict__ R,
+                               double* __restrict__ Bt) {
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s || c >= s) return;
+  Bt[(size_t)c * s + i] = (i < rn && c < r) ? R[(size_t)c * r + i] : 0.0;
+}
+void b_block(hipStream_t st, int s, int r, int rn, const double* R, double* Bt) {
+  dim3 blk(16, 16), grid((s + 15) / 16, (s + 15) / 16);
+  hipLaunchKernelGGL(b_block_kernel, grid, blk, 0, st, s, r, rn, R, Bt);
   SMALGO_HIP(hipGetLastError());
 }
 

@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <limits>
+#include <map>
 #include <memory>
 #include <numeric>
 
@@ -45,6 +46,10 @@ int Dense::potrf_upper(int b, double* A, int lda) {
   return h;
 }
 
+void Dense::potrf_upper_async(int b, double* A, int lda, int* dinfo) {
+  check_rb(rocsolver_dpotrf(cx.blas, rocblas_fill_upper, b, A, lda, dinfo), "rocsolver_dpotrf");
+}
+
 void Dense::trtri_upper(int b, double* A, int lda) {
   check_rb(rocsolver_dtrtri(cx.blas, rocblas_fill_upper, rocblas_diagonal_non_unit, b, A, lda,
                             info.p),
@@ -60,6 +65,76 @@ void Dense::syevd(int n, double* A, int lda, double* w, double* work_e, bool upp
   SMALGO_HIP(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, cx.stream));
   SMALGO_HIP(hipStreamSynchronize(cx.stream));
   if (h != 0) throw Error(-4, "rocsolver_dsyevd did not converge (info=" + std::to_string(h) + ")");
+}
+
+// ---------------------------------------------------------------------------
+// device memory cache (per thread and device)
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kPoolCacheBytes = size_t(24) << 30;
+struct PoolCache {
+  std::multimap<size_t, void*> free_blocks[64];
+  size_t cached[64] = {};
+  ~PoolCache() {
+    for (int d = 0; d < 64; ++d)
+      for (auto& e : free_blocks[d]) {
+        if (hipSetDevice(d) == hipSuccess) (void)hipFree(e.second);
+      }
+  }
+};
+thread_local PoolCache g_pool;
+// requests round up to 2 MiB above 2 MiB (256 B below), so blocks of nearby
+// sizes are interchangeable
+size_t pool_round(size_t b) {
+  const size_t g = b > (size_t(2) << 20) ? (size_t(2) << 20) : 256;
+  return (b + g - 1) / g * g;
+}
+}  // namespace
+
+void* pool_alloc(size_t bytes, int* device) {
+  int d = 0;
+  SMALGO_HIP(hipGetDevice(&d));
+  *device = d;
+  const size_t b = pool_round(bytes);
+  if (d >= 0 && d < 64) {
+    auto& fl = g_pool.free_blocks[d];
+    auto it = fl.lower_bound(b);
+    if (it != fl.end() && it->first <= b + b / 4) {
+      void* p = it->second;
+      g_pool.cached[d] -= it->first;
+      fl.erase(it);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess && d >= 0 && d < 64 && !g_pool.free_blocks[d].empty()) {
+    (void)hipGetLastError();
+    for (auto& f : g_pool.free_blocks[d]) (void)hipFree(f.second);  // give the cache back
+    g_pool.free_blocks[d].clear();
+    g_pool.cached[d] = 0;
+    e = hipMalloc(&p, b);
+  }
+  SMALGO_HIP(e);
+  return p;
+}
+
+void pool_free(void* p, size_t bytes, int device) {
+  if (!p) return;
+  if (device < 0 || device >= 64) {
+    (void)hipFree(p);
+    return;
+  }
+  const size_t b = pool_round(bytes);
+  auto& fl = g_pool.free_blocks[device];
+  fl.emplace(b, p);
+  g_pool.cached[device] += b;
+  while (g_pool.cached[device] > kPoolCacheBytes && !fl.empty()) {
+    auto last = std::prev(fl.end());  // the largest cached block goes back
+    g_pool.cached[device] -= last->first;
+    (void)hipFree(last->second);
+    fl.erase(last);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -293,30 +368,38 @@ static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ld
     for (int l : {4, 8, 16, 32, 64}) {
       const int w = l * v;  // chunk width in elements
       if (w > 2 * std::max(b, 4) || w * (int)sizeof(T) > 2048) continue;
-      for (int rt : {1, 2, 4}) cand.push_back(FilterTile{v, l, rt});
+      for (int rt : {1, 2, 4})
+        for (int lds : {0, 1}) cand.push_back(FilterTile{v, l, rt, lds});
     }
   }
   if (cand.empty()) return def;
   Event e0, e1;
-  FilterTile best = def;
-  float best_ms = 1e30f;
-  for (const FilterTile& t : cand) {
-    // warm, then time 4 steps of the general form out = cq Q + a1 A X1 + b1 X1 - X2
-    launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), Q, ldq, T(0.1), T(0.1), buf1,
-                            ldb, T(1), buf0, ldb, t);
-    SMALGO_HIP(hipEventRecord(e0, cx.stream));
-    for (int r = 0; r < 4; ++r)
+  // Two sweeps over the candidates, each timing 6 steps of the general form
+  // out = cq Q + a1 A X1 + b1 X1 - X2 after one warm step; a tile's time is
+  // its faster sweep, so clock ramps and one-off stalls do not pick the tile.
+  std::vector<float> tmin(cand.size(), 1e30f);
+  for (int sweep = 0; sweep < 2; ++sweep)
+    for (size_t i = 0; i < cand.size(); ++i) {
+      const FilterTile& t = cand[i];
       launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), Q, ldq, T(0.1), T(0.1), buf1,
                               ldb, T(1), buf0, ldb, t);
-    SMALGO_HIP(hipEventRecord(e1, cx.stream));
-    SMALGO_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = t;
+      SMALGO_HIP(hipEventRecord(e0, cx.stream));
+      for (int r = 0; r < 6; ++r)
+        launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), Q, ldq, T(0.1), T(0.1), buf1,
+                                ldb, T(1), buf0, ldb, t);
+      SMALGO_HIP(hipEventRecord(e1, cx.stream));
+      SMALGO_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
+      tmin[i] = std::min(tmin[i], ms);
     }
-  }
+  FilterTile best = def;
+  float best_ms = 1e30f;
+  for (size_t i = 0; i < cand.size(); ++i)
+    if (tmin[i] < best_ms) {
+      best_ms = tmin[i];
+      best = cand[i];
+    }
   cx.tuned[key] = best;
   return best;
 }
@@ -374,9 +457,9 @@ static int cache_panel_width(const Op& op) {
 static bool env_tile(const char* name, FilterTile* t) {
   const char* v = std::getenv(name);
   if (!v) return false;
-  int a = 0, b = 0, c = 1;
-  if (std::sscanf(v, "%d,%d,%d", &a, &b, &c) < 2) return false;
-  *t = FilterTile{a, b, c};
+  int a = 0, b = 0, c = 1, d = 0;
+  if (std::sscanf(v, "%d,%d,%d,%d", &a, &b, &c, &d) < 2) return false;
+  *t = FilterTile{a, b, c, d};
   return true;
 }
 
@@ -509,8 +592,9 @@ void apply_filter_noalloc(Ctx& cx, const Op& op, int b, const double* Q, int ldq
 // the caller keeps the first r (the reference's rule, polfed.jl:679-684).
 // ---------------------------------------------------------------------------
 struct OrthWork {
-  DBuf<double> G, Rinv, gemm_work, cn, red;
-  std::vector<double> hG;
+  DBuf<double> G, Rinv, gemm_work, cn, red, R1, status;
+  DBuf<int> info2;
+  std::vector<double> hG, hstatus;
 };
 
 template <typename T>
@@ -610,6 +694,55 @@ static int cholqr(Ctx& cx, Dense& dn, OrthWork& ow, int n, int b, const T* W, in
   if (r0 > 0)
     for (int i = 0; i < b; ++i) r += std::fabs(dacc[i]) > rtol * r0 ? 1 : 0;
   if (Rout) *Rout = std::move(Racc);
+  return r;
+}
+
+// CholQR2 of block Lanczos without host round trips: both passes, the
+// product R = R_2·R_1 and the status stay on the device, and one small
+// download (the two Cholesky infos and diag(R)) gives the rank.  Returns -1
+// when a Cholesky factorization failed; the caller then runs cholqr(), whose
+// shifted CholQR3 handles a Gram matrix that is not numerically positive
+// definite.  W is only read, so that retry starts from the same block.
+template <typename T>
+static int cholqr_fast(Ctx& cx, Dense& dn, OrthWork& ow, int n, int b, const T* W, int ldw,
+                       T* Qout, int ldq, T* tmp, int ldt, double rtol, double* Rdev) {
+  ow.G.ensure((size_t)b * b);
+  ow.R1.ensure((size_t)b * b);
+  ow.gemm_work.ensure(gemm_tn_work_elems(n, b, b));
+  ow.info2.ensure(2);
+  ow.status.ensure((size_t)b + 2);
+  ow.hstatus.resize((size_t)b + 2);
+  // pass 1: G = WᵀW = R_1ᵀR_1, tmp = W R_1⁻¹
+  gemm_tn<T, T, double>(cx.stream, n, b, b, 1.0, W, ldw, W, ldw, 0.0, ow.G.p, b, ow.gemm_work.p,
+                        ow.gemm_work.n);
+  dn.potrf_upper_async(b, ow.G.p, b, ow.info2.p);
+  zero_lower_cm(cx.stream, b, ow.G.p, b);
+  SMALGO_HIP(hipMemcpyAsync(ow.R1.p, ow.G.p, sizeof(double) * b * b, hipMemcpyDeviceToDevice,
+                            cx.stream));
+  dn.trtri_upper(b, ow.G.p, b);
+  gemm_nn<T, double, T>(cx.stream, n, b, b, 1.0, W, ldw, ow.G.p, b, true, 0.0, tmp, ldt);
+  // pass 2: tmpᵀtmp = R_2ᵀR_2, Q = tmp R_2⁻¹, R = R_2 R_1
+  gemm_tn<T, T, double>(cx.stream, n, b, b, 1.0, tmp, ldt, tmp, ldt, 0.0, ow.G.p, b,
+                        ow.gemm_work.p, ow.gemm_work.n);
+  dn.potrf_upper_async(b, ow.G.p, b, ow.info2.p + 1);
+  zero_lower_cm(cx.stream, b, ow.G.p, b);
+  tri_mul_upper(cx.stream, b, ow.G.p, ow.R1.p, Rdev);
+  dn.trtri_upper(b, ow.G.p, b);
+  gemm_nn<T, double, T>(cx.stream, n, b, b, 1.0, tmp, ldt, ow.G.p, b, true, 0.0, Qout, ldq);
+  pack_qr_status(cx.stream, b, ow.info2.p, Rdev, ow.status.p);
+  SMALGO_HIP(hipMemcpyAsync(ow.hstatus.data(), ow.status.p, sizeof(double) * (b + 2),
+                            hipMemcpyDeviceToHost, cx.stream));
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  if (ow.hstatus[0] != 0.0 || ow.hstatus[1] != 0.0) return -1;
+  double r0 = 0.0;
+  for (int i = 0; i < b; ++i) {
+    const double d = std::fabs(ow.hstatus[2 + i]);
+    if (!std::isfinite(d)) return -1;
+    r0 = std::max(r0, d);
+  }
+  int r = 0;
+  if (r0 > 0)
+    for (int i = 0; i < b; ++i) r += std::fabs(ow.hstatus[2 + i]) > rtol * r0 ? 1 : 0;
   return r;
 }
 
@@ -914,7 +1047,7 @@ Result* subspace_rr_host(Ctx& cx, const Op& op, int k, const double* Y, int ldy,
                               cx.stream));
     SMALGO_HIP(hipMemcpyAsync(out->res.data(), resd.p, sizeof(double) * k,
                               hipMemcpyDeviceToHost, cx.stream));
-    SMALGO_HIP(hipMalloc((void**)&out->Xdev, sizeof(double) * (size_t)n * k));
+    out->alloc_X(sizeof(double) * (size_t)n * k);
     rm_to_cm<double>(cx.stream, n, k, X.p, ld, nullptr, out->Xdev, n);
     if (Qh) download_cm<double>(cx, n, k, Q.p, ld, Qh, ldqh);
     SMALGO_HIP(hipStreamSynchronize(cx.stream));
@@ -936,7 +1069,7 @@ Result* refine_rr_host(Ctx& cx, const Op& op, int k, const double* X, int ldx) {
   out->device = cx.device;
   const int kq = refine_rr(cx, dn, op, n, k, dX.p, lk, w, Y, ldy, out->lam, out->res);
   if (kq > 0) {
-    SMALGO_HIP(hipMalloc((void**)&out->Xdev, sizeof(double) * (size_t)n * kq));
+    out->alloc_X(sizeof(double) * (size_t)n * kq);
     rm_to_cm<double>(cx.stream, n, kq, Y.p, ldy, nullptr, out->Xdev, n);
     SMALGO_HIP(hipStreamSynchronize(cx.stream));
   }
@@ -1203,7 +1336,8 @@ struct SliceFilter {
       for (int v : {1, 2, 4, 8})
         for (int l : {4, 8, 16, 32, 64})
           if (w % v == 0 && l * v <= std::max(w, 4) * 2 && l * v * (int)sizeof(T) <= 2048)
-            for (int rt : {1, 2, 4}) cand.push_back(FilterTile{v, l, rt});
+            for (int rt : {1, 2, 4})
+              for (int lds : {0, 1}) cand.push_back(FilterTile{v, l, rt, lds});
     }
     Event e0, e1;
     FilterTile best = cand.front();
@@ -1387,6 +1521,10 @@ struct Basis {
   std::vector<int> ranks;
   std::vector<std::vector<double>> Ablk, Bblk;  // column-major r_j x r_j, r_{j+1} x r_j
   DBuf<double> Adev, Bprev;                     // s x s device blocks
+  // every block's A_j (r_j x r_j) and R_j (r_j x r_j, B_j = its first r_{j+1}
+  // rows), column-major at s·s stride: read by the host once, after the basis
+  DBuf<double> Aall, Rall;
+  std::vector<int> rnext;  // r_{j+1} of each stored R_j
   int r_cur = 0, r_prev = 0;
   bool done = false;
 };
@@ -1425,6 +1563,36 @@ static void basis_alloc(Basis<T>& B, int n, int s, int mmax) {
   B.V.alloc((size_t)n * B.ldV);
   B.Adev.alloc((size_t)s * s);
   B.Bprev.alloc((size_t)s * s);
+  B.Aall.alloc((size_t)mmax * s * s);
+  B.Rall.alloc((size_t)mmax * s * s);
+}
+
+// The reduced-matrix blocks to the host (one download after the basis is built)
+template <typename T>
+static void basis_download(Ctx& cx, Basis<T>& B) {
+  const int nb = (int)B.ranks.size(), ss = B.s * B.s;
+  std::vector<double> a((size_t)nb * ss), r((size_t)B.rnext.size() * ss);
+  if (nb)
+    SMALGO_HIP(hipMemcpyAsync(a.data(), B.Aall.p, sizeof(double) * a.size(),
+                              hipMemcpyDeviceToHost, cx.stream));
+  if (!r.empty())
+    SMALGO_HIP(hipMemcpyAsync(r.data(), B.Rall.p, sizeof(double) * r.size(),
+                              hipMemcpyDeviceToHost, cx.stream));
+  SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  B.Ablk.assign(nb, {});
+  for (int j = 0; j < nb; ++j) {
+    const int rj = B.ranks[j];
+    B.Ablk[j].assign(a.begin() + (size_t)j * ss, a.begin() + (size_t)j * ss + (size_t)rj * rj);
+  }
+  B.Bblk.assign(B.rnext.size(), {});
+  for (size_t j = 0; j < B.rnext.size(); ++j) {
+    const int rj = B.ranks[j], rn = B.rnext[j];
+    const double* R = r.data() + j * ss;  // r_j x r_j, ld r_j
+    std::vector<double> Bj((size_t)rn * rj);
+    for (int c = 0; c < rj; ++c)
+      for (int i = 0; i < rn; ++i) Bj[(size_t)c * rn + i] = R[(size_t)c * rj + i];
+    B.Bblk[j] = std::move(Bj);
+  }
 }
 
 // One block-Lanczos step after the filter W = p(Â) Q_j (polfed.jl:1066-1097):
@@ -1454,11 +1622,9 @@ static void lanczos_step(Ctx& cx, Dense& dn, StepWork& sw, const smalgo_polfed_p
                         sw.gw.n);
   symmetrize(cx.stream, r, B.Adev.p, r);
   gemm_flops += 2.0 * n * r * r;
-  std::vector<double> Aj((size_t)r * r);
-  SMALGO_HIP(hipMemcpyAsync(Aj.data(), B.Adev.p, sizeof(double) * r * r, hipMemcpyDeviceToHost,
-                            cx.stream));
-  SMALGO_HIP(hipStreamSynchronize(cx.stream));
-  B.Ablk.push_back(std::move(Aj));
+  // A_j stays on the device; the host reads all blocks after the basis
+  SMALGO_HIP(hipMemcpyAsync(B.Aall.p + (size_t)j * s * s, B.Adev.p, sizeof(double) * r * r,
+                            hipMemcpyDeviceToDevice, cx.stream));
   B.ranks.push_back(r);
   if (j == B.mmax - 1) {  // Q_{mmax+1} and B_mmax never enter T or the Ritz vectors
     B.done = true;
@@ -1480,26 +1646,26 @@ static void lanczos_step(Ctx& cx, Dense& dn, StepWork& sw, const smalgo_polfed_p
     gemm_nn<T, T, T>(cx.stream, n, r, r, -1.0, Qj, ldV, Cbuf.p, ldc, false, 1.0, W, ldw);
     gemm_flops += 4.0 * n * mall * r + 4.0 * n * r * r;
   }
-  // [Q_{j+1}, B_j] = block_qr_factor(W; rtol)   (polfed.jl:1090)
+  // [Q_{j+1}, B_j] = block_qr_factor(W; rtol)   (polfed.jl:1090): CholQR2 on the
+  // device, R_j (r x r) straight into its slot of Rall
   T* Qn = B.V.p + (size_t)(j + 1) * s;
-  const int rn = cholqr<T>(cx, dn, sw.ow, n, r, W, ldw, Qn, ldV, tmp, ldt, p.rtol, &sw.R);
+  double* Rj = B.Rall.p + (size_t)j * s * s;
+  int rn = cholqr_fast<T>(cx, dn, sw.ow, n, r, W, ldw, Qn, ldV, tmp, ldt, p.rtol, Rj);
+  if (rn < 0) {  // a Cholesky factorization failed: the shifted path, R from the host
+    rn = cholqr<T>(cx, dn, sw.ow, n, r, W, ldw, Qn, ldV, tmp, ldt, p.rtol, &sw.R);
+    if (rn > 0)
+      SMALGO_HIP(hipMemcpyAsync(Rj, sw.R.data(), sizeof(double) * r * r, hipMemcpyHostToDevice,
+                                cx.stream));
+  }
   gemm_flops += 8.0 * n * r * r;
   if (rn == 0) {  // rank collapse (polfed.jl:1091)
     B.done = true;
     return;
   }
   if (rn < s) fill_block<T>(cx.stream, n, s - rn, Qn + rn, ldV, T(0));
-  // B_j = R[1:rn, 1:r] (column-major); device copy readable as Bᵀ: element (k, i) at k*s + i
-  std::vector<double> Bj((size_t)rn * r), Bt((size_t)s * s, 0.0);
-  for (int c = 0; c < r; ++c)
-    for (int i = 0; i < rn; ++i) {
-      Bj[(size_t)c * rn + i] = sw.R[(size_t)c * r + i];
-      Bt[(size_t)c * s + i] = sw.R[(size_t)c * r + i];
-    }
-  SMALGO_HIP(hipMemcpyAsync(B.Bprev.p, Bt.data(), sizeof(double) * s * s, hipMemcpyHostToDevice,
-                            cx.stream));
-  SMALGO_HIP(hipStreamSynchronize(cx.stream));
-  B.Bblk.push_back(std::move(Bj));
+  // B_j = R[1:rn, 1:r]; the device copy the next step reads as Bᵀ: element (k, i) at k*s + i
+  b_block(cx.stream, s, r, rn, Rj, B.Bprev.p);
+  B.rnext.push_back(rn);
   B.r_prev = r;
   B.r_cur = rn;
 }
@@ -1929,7 +2095,7 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
     cols[i] = final_cols[pick[i]];
   }
   if (kout > 0) {
-    SMALGO_HIP(hipMalloc((void**)&out->Xdev, sizeof(double) * (size_t)n * kout));
+    out->alloc_X(sizeof(double) * (size_t)n * kout);
     DBuf<int> dc = upload_idx(cx, cols);
     rm_to_cm<double>(cx.stream, n, kout, Xfin.p, ldxf, dc.p, out->Xdev, n);
     SMALGO_HIP(hipStreamSynchronize(cx.stream));
@@ -1982,6 +2148,11 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
   int basis_lps = 1;
   long long filter_steps = 0;
   double gemm_flops = 0;
+  // algorithmic bytes of the basis-build filter, per fused step and launch:
+  // nnz*(w+4) + 4(n+1) for the matrix (read once per launch: once per panel
+  // when a wide block is split) + 4*n*b*w for the four block streams
+  double bytes = 0;
+  const double mat_bytes = (double)(op.A ? op.A->nnz : 0) * (sizeof(T) + 4) + 4.0 * (n + 1);
   for (int j = 0; j < mmax && !B.done; ++j) {
     T* Qj = B.V.p + (size_t)j * s;
     size_t hf = tm.begin("filter");
@@ -1993,11 +2164,14 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
       basis_lps = cx.last_launches_per_step;
     }
     filter_steps += degree;  // K fused steps (the all-zero first step is skipped)
+    bytes += (double)degree * (cx.last_launches_per_step * mat_bytes +
+                               4.0 * n * B.r_cur * sizeof(T));
     size_t ho = tm.begin("orth");
     lanczos_step<T>(cx, dn, sw, p, n, B, j, W.p, lds, b0.p, lds, Cbuf, gemm_flops);
     tm.end(ho);
   }
   ExtractStats xs;
+  basis_download<T>(cx, B);
   std::unique_ptr<Result> out(polfed_extract<T>(cx, dn, tm, op, B, nev, degree, smin, smax, p, xs));
   auto ph = tm.resolve();
   const int nb = (int)B.ranks.size();
@@ -2018,12 +2192,7 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
   out->stats["tile_vec"] = basis_tile.vec;
   out->stats["tile_lpr"] = basis_tile.lpr;
   out->stats["tile_rt"] = basis_tile.rt;
-  // algorithmic bytes of the basis-build filter: per fused step
-  // nnz*(w+4) + 4(n+1) + 4*n*b*w   (b = block width, w = sizeof(T))
-  double bytes = 0;
-  for (int j = 0; j < nb; ++j)
-    bytes += (double)degree * ((double)(op.A ? op.A->nnz : 0) * (sizeof(T) + 4) + 4.0 * (n + 1) +
-                               4.0 * n * B.ranks[j] * sizeof(T));
+  out->stats["tile_lds"] = basis_tile.lds;
   out->stats["spmm_bytes"] = bytes;
   return out.release();
 }
@@ -2182,7 +2351,7 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
     out->res.push_back(res[idx[i]]);
   }
   if (!cols.empty()) {
-    SMALGO_HIP(hipMalloc((void**)&out->Xdev, sizeof(double) * (size_t)n * cols.size()));
+    out->alloc_X(sizeof(double) * (size_t)n * cols.size());
     DBuf<int> dc = upload_idx(cx, cols);
     rm_to_cm<double>(cx.stream, n, (int)cols.size(), X.p, ld, dc.p, out->Xdev, n);
   }
@@ -2198,10 +2367,14 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
   out->stats["tile_vec"] = cheb_tile.vec;
   out->stats["tile_lpr"] = cheb_tile.lpr;
   out->stats["tile_rt"] = cheb_tile.rt;
+  out->stats["tile_lds"] = cheb_tile.lds;
   out->stats["smin"] = smin;
   out->stats["smax"] = smax;
-  out->stats["spmm_bytes"] = (double)filter_steps * ((double)(op.A ? op.A->nnz : 0) * 12.0 +
-                                                    4.0 * (n + 1) + 4.0 * n * m0 * 8.0);
+  // the matrix is read once per launch (once per panel of a wide block)
+  out->stats["spmm_bytes"] = (double)filter_steps *
+                             ((op.is_callback() ? 1 : cheb_lps) *
+                                  ((double)(op.A ? op.A->nnz : 0) * 12.0 + 4.0 * (n + 1)) +
+                              4.0 * n * m0 * 8.0);
   char buf[64];
   std::snprintf(buf, sizeof buf, "it=%d inside=%zu", its, out->lam.size());
   out->line = buf;
@@ -2386,6 +2559,7 @@ static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
       pp.mmax = mmax[i];
       pp.seed = p.seed + (uint64_t)i;
       ExtractStats xs;
+      basis_download<T>(cx, B[i]);
       out[i] = polfed_extract<T>(cx, dn, tm, op, B[i], sl[i].nev, sl[i].degree, smin, smax, pp, xs);
       out[i]->stats["filter_steps"] = (double)fsteps[i];
       out[i]->stats["refine_filter_steps"] = (double)xs.refine_steps;
@@ -2425,6 +2599,7 @@ static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
     r->stats["tile_vec"] = wt.vec;
     r->stats["tile_lpr"] = wt.lpr;
     r->stats["tile_rt"] = wt.rt;
+    r->stats["tile_lds"] = wt.lds;
   }
   return out;
 }

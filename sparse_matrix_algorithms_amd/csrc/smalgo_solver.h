@@ -12,28 +12,38 @@
 
 namespace smalgo {
 
+// Device memory of the solvers comes from a per-thread, per-device cache of
+// freed blocks: a solve's workspace is handed back to the next solve instead
+// of going through hipFree/hipMalloc (each of which synchronises the device).
+// Safe because every entry point finishes its work on its one stream before
+// it returns, so a returned block is idle, and blocks never move between
+// threads.  At most kPoolCacheBytes stay cached per thread and device.
+void* pool_alloc(size_t bytes, int* device);
+void pool_free(void* p, size_t bytes, int device);
+
 // RAII device buffer
 template <typename T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  int dev = 0;
   DBuf() = default;
   explicit DBuf(size_t count) { alloc(count); }
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
-  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n), dev(o.dev) { o.p = nullptr; o.n = 0; }
   DBuf& operator=(DBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    if (this != &o) { release(); p = o.p; n = o.n; dev = o.dev; o.p = nullptr; o.n = 0; }
     return *this;
   }
   void alloc(size_t count) {
     release();
-    if (count) SMALGO_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    if (count) p = static_cast<T*>(pool_alloc(count * sizeof(T), &dev));
     n = count;
   }
   void ensure(size_t count) { if (count > n) alloc(count); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) pool_free(p, n * sizeof(T), dev);
     p = nullptr;
     n = 0;
   }
@@ -98,12 +108,20 @@ struct Op {
 struct Result {
   int64_t n = 0;
   std::vector<double> lam, res;
-  double* Xdev = nullptr;  // column-major n x k on the device
+  double* Xdev = nullptr;  // column-major n x k on the device (from pool_alloc)
+  size_t Xbytes = 0;
   int device = 0;
   std::map<std::string, double> stats;
   std::string line;
+  void alloc_X(size_t bytes) { Xdev = static_cast<double*>(pool_alloc(bytes, &device)); Xbytes = bytes; }
   ~Result() {
-    if (Xdev) (void)hipFree(Xdev);
+    if (Xdev) {
+      int cur = 0;
+      const bool sw = hipGetDevice(&cur) == hipSuccess && cur != device;
+      if (sw) (void)hipSetDevice(device);
+      pool_free(Xdev, Xbytes, device);
+      if (sw) (void)hipSetDevice(cur);
+    }
   }
 };
 
@@ -113,6 +131,8 @@ struct Dense {
   DBuf<int> info;
   explicit Dense(Ctx& c) : cx(c), info(1) {}
   int potrf_upper(int b, double* A, int lda);  // returns info (0 = success)
+  // the same without waiting: info is written to dinfo on the device
+  void potrf_upper_async(int b, double* A, int lda, int* dinfo);
   void trtri_upper(int b, double* A, int lda);
   void syevd(int n, double* A, int lda, double* w, double* work_e, bool upper);
 };

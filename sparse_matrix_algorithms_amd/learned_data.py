@@ -226,9 +226,95 @@ def evaluate(path: Path, weights: Path = None, margin_kappa=None, margin_factor=
     return summ, rows
 
 
+# constant (κ, f) rules tried against the learned one (the plain rule is (8, 2.8))
+CONST_RULES = [(5.0, 2.8), (6.0, 2.8), (6.0, 3.4), (6.0, 4.2), (8.0, 3.4), (8.0, 4.2)]
+# (degree scale, basis scale) candidates of the mixed path's learned parameters
+MIXED_SCALES = [(0.6, 1.08), (0.7, 1.0), (0.7, 1.08), (0.8, 1.0), (0.8, 1.08), (1.0, 1.0)]
+
+
+def _rule_params(rule, model, r):
+    """(degree, mmax, precision, source) of one rule on record r: ("plain",),
+    ("const", κ, f) and ("learned",) in FP64, or ("mixed", degree scale, basis
+    scale) for the learned parameters on the mixed path."""
+    from sparse_matrix_algorithms_amd import kpm
+    from sparse_matrix_algorithms_amd.learned import choose_parameters
+    if rule[0] == "plain":
+        return r["plain"]["degree"], r["plain"]["mmax"], "fp64", "plain"
+    if rule[0] == "const":
+        K, mm = kpm.plain_parameters(r["n"], r["nev"], r["emin"], r["emax"], r["smin"], r["smax"],
+                                     S, rule[1], rule[2])
+        return K, mm, "fp64", "const"
+    K, mm, src = choose_parameters(model, np.array(r["mu"]), r["n"], r["nev"], r["emin"],
+                                   r["emax"], r["smin"], r["smax"], S)
+    if rule[0] == "learned":
+        return K, mm, "fp64", src
+    if src != "fallback":
+        K = max(1, math.ceil(rule[1] * K))
+        mm = max(1, math.ceil(rule[2] * mm - 1e-9))
+    return K, mm, "fp32", src
+
+
+def compare_rules(path: Path, rules, first: int, last: int, out: Path):
+    """Solve records [first, last) with every rule and with the plain rule as
+    the reference set; one JSON line per record (appended to `out`), with the
+    time, the filter steps, the valid pairs and success per rule.  Success is
+    the held-out evaluation's: at least 0.9 of the KPM count at relative
+    residual <= 1e-10, covering the plain run's set where that run converged."""
+    import sparse_matrix_algorithms_amd as sm
+    from sparse_matrix_algorithms_amd.learned import MLP
+    from sparse_matrix_algorithms_amd.lattices import anderson3d
+    ctx = sm.default_context()
+    model = MLP.load(ctx=ctx)
+    done = set()
+    if out.exists():
+        done = {json.loads(l)["index"] for l in open(out) if l.strip()}
+    with open(out, "a") as f:
+        for r in [r for r in load(path) if first <= r["index"] < last and r["index"] not in done]:
+            A = anderson3d(r["L"], r["W"], r["seed"])
+            dm = sm.DeviceMatrix(A, ctx)
+            anorm = max(abs(r["smin"]), abs(r["smax"]))
+            need = 0.9 * r["count_est"]
+            row = dict(index=r["index"], nev=r["nev"], count_est=r["count_est"], rules={})
+            ref = None
+            for rule in [("plain",)] + list(rules):
+                K, mm, prec, src = _rule_params(rule, model, r)
+                lam, t, st = _solve(sm, dm, r["nev"], K, mm, r["emin"], r["emax"], r["smin"],
+                                    r["smax"], anorm, ctx, prec)
+                if ref is None:
+                    ref = lam if len(lam) >= need else np.zeros(0)
+                covered = len(ref) == 0 or (len(lam) > 0 and float(np.max(
+                    np.abs(ref[:, None] - lam[None, :]).min(axis=1))) <= 1e-10 * anorm)
+                row["rules"]["/".join(map(str, rule))] = dict(
+                    ok=bool(len(lam) >= need and covered), seconds=t, degree=K, mmax=mm,
+                    source=src, pairs=int(len(lam)),
+                    filter_steps=float(st["filter_steps"]),
+                    refine_column_steps=float(st.get("refine_filter_steps", 0.0)))
+            f.write(json.dumps(row) + "\n")
+            f.flush()
+            print(f"[rules] {r['index']} " + " ".join(
+                f"{k}:{'ok' if v['ok'] else 'FAIL'}:{v['seconds']:.3f}s"
+                for k, v in row["rules"].items()), flush=True)
+            dm.free()
+
+
+def summarize_rules(path: Path):
+    """Per rule over the records of a compare_rules log: successes, total
+    seconds, eigenpairs/s (pairs of successful runs over all seconds) and
+    filter steps."""
+    rows = [json.loads(l) for l in open(path) if l.strip()]
+    out = {}
+    for k in rows[0]["rules"]:
+        rr = [x["rules"][k] for x in rows]
+        t = sum(x["seconds"] for x in rr)
+        out[k] = dict(records=len(rr), success=sum(x["ok"] for x in rr), seconds=t,
+                      eigenpairs_per_s=sum(x["pairs"] for x in rr if x["ok"]) / max(t, 1e-9),
+                      filter_steps=sum(x["filter_steps"] for x in rr))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["gen", "train", "eval"])
+    ap.add_argument("cmd", choices=["gen", "train", "eval", "rules", "summary"])
     ap.add_argument("--first", type=int, default=0)
     ap.add_argument("--last", type=int, default=N_TOTAL)
     ap.add_argument("--data", type=Path, default=DATA / "anderson_runs.jsonl")
@@ -245,6 +331,11 @@ def main():
                     help="eval: factor on the candidate's basis size")
     ap.add_argument("--rule", choices=["learned", "plain"], default="learned",
                     help="eval: parameters of the candidate runs")
+    ap.add_argument("--set", choices=["const", "mixed", "heldout"], default="const",
+                    help="rules: the constant (κ, f) rules, the mixed-path scales, or the "
+                         "held-out comparison (--const: the chosen constant rules)")
+    ap.add_argument("--const", action="append", default=[],
+                    help="rules --set heldout: a constant rule 'kappa,factor' (repeatable)")
     ap.add_argument("--redo-unconverged", type=Path, default=None,
                     help="regenerate the instances of this log whose plain run did not converge")
     a = ap.parse_args()
@@ -257,6 +348,19 @@ def main():
         gen(a.first, a.last, a.out or a.data, idx)
     elif a.cmd == "train":
         train(a.data, a.epochs, a.lr, a.out)
+    elif a.cmd == "rules":
+        if a.set == "const":
+            rules = [("const",) + r for r in CONST_RULES]
+        elif a.set == "mixed":
+            rules = [("mixed",) + r for r in MIXED_SCALES]
+        else:
+            from sparse_matrix_algorithms_amd.learned import MIXED_DEGREE_SCALE, MIXED_MMAX_SCALE
+            rules = [("learned",)] + [("const",) + tuple(float(x) for x in c.split(","))
+                                       for c in a.const]
+            rules.append(("mixed", MIXED_DEGREE_SCALE, MIXED_MMAX_SCALE))
+        compare_rules(a.data, rules, a.first, a.last, a.out)
+    elif a.cmd == "summary":
+        print(json.dumps(summarize_rules(a.out), indent=1))
     else:
         summ, rows = evaluate(a.data, precision=a.precision, degree_scale=a.degree_scale,
                               refine_scale=a.refine_scale, rule=a.rule, mmax_scale=a.mmax_scale)

@@ -66,3 +66,15 @@ def test_c_restatement_matches_numpy():
     Y = clenshaw_c.apply_poly_block(A, Q, c, smin, smax, nthreads=2)
     Yo = P.apply_poly_block(A, Q, c, smin, smax)
     assert np.max(np.abs(Y - Yo)) <= 1e-13 * np.abs(Yo).max()
+
+
+def test_c_rowmajor_restatement_matches_numpy():
+    from oracle import clenshaw_c
+    A = anderson3d(6, 16.5, 5)
+    smin, smax = P.gershgorin_bounds(A)
+    Q = gaussian_block(4, A.shape[0], 7)
+    for K in (0, 1, 2, 50):
+        c = P.build_coeffs(K, smin, smax, 0.0, -0.4, 0.4, "step")
+        Y = clenshaw_c.apply_poly_block_rowmajor(A, Q, c, smin, smax, nthreads=2)
+        Yo = P.apply_poly_block(A, Q, c, smin, smax)
+        assert np.max(np.abs(Y - Yo)) <= 1e-13 * np.abs(Yo).max(), K
