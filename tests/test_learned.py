@@ -105,3 +105,34 @@ def test_mixed_path_scales():
                                      precision="fp32")[:2] == (Kq, mq)
     with pytest.raises(ValueError):
         plain_parameters(n, nev, emin, emax, smin, smax, s, precision="bf16")
+
+
+def test_committed_rule_comparisons():
+    """Training: constant (κ, f) rules solved on all 256 training records
+    (data/constant_rules_train.jsonl); held-out: the learned parameters, the
+    plain rule, the training's best constant rules and the mixed path on the
+    64 held-out instances (data/heldout_rules.jsonl), all on the GPU."""
+    from sparse_matrix_algorithms_amd.learned_data import summarize_rules
+    tr = summarize_rules(learned.WEIGHTS.parent / "constant_rules_train.jsonl")
+    assert all(v["records"] == 256 for v in tr.values())
+    best = max((k for k in tr if k.startswith("const/")), key=lambda k: tr[k]["eigenpairs_per_s"])
+    assert best == "const/6.0/3.4"
+    ho = summarize_rules(learned.WEIGHTS.parent / "heldout_rules.jsonl")
+    assert all(v["records"] == 64 for v in ho.values())
+    # the network converges more held-out instances than any rule it replaces
+    for k in ho:
+        if k.startswith("const/") or k == "plain":
+            assert ho["learned"]["success"] > ho[k]["success"] or k == "const/6.0/4.2", k
+    assert ho["learned"]["success"] >= ho["const/6.0/4.2"]["success"]
+
+
+def test_mixed_scales_chosen_on_training_records():
+    """The mixed path's (degree, basis) factors are the best of the six pairs
+    solved on the 256 training records (most successes, then eigenpairs/s);
+    no held-out instance entered the choice."""
+    from sparse_matrix_algorithms_amd.learned_data import MIXED_SCALES, summarize_rules
+    tr = summarize_rules(learned.WEIGHTS.parent / "mixed_scales_train.jsonl")
+    keys = [f"mixed/{d}/{m}" for d, m in MIXED_SCALES]
+    assert set(keys) <= set(tr) and all(tr[k]["records"] == 256 for k in keys)
+    best = max(keys, key=lambda k: (tr[k]["success"], tr[k]["eigenpairs_per_s"]))
+    assert best == f"mixed/{learned.MIXED_DEGREE_SCALE}/{learned.MIXED_MMAX_SCALE}"
