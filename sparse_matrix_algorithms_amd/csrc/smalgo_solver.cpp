@@ -2139,8 +2139,7 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
   StepWork sw;
   Basis<T> B;
   basis_alloc<T>(B, n, s, mmax);
-  DBuf<T> W((size_t)n * lds), b0((size_t)n * lds), b1((size_t)n * lds), Qc((size_t)n * lds),
-      Cbuf;
+  DBuf<T> W((size_t)n * lds), b0((size_t)n * lds), b1((size_t)n * lds), Cbuf;
   DBuf<double> pbuf;
   if (op.is_callback()) pbuf.alloc((size_t)n * lds);
   fill_block<T>(cx.stream, n, B.ldV, B.V.p, B.ldV, T(0));
@@ -2157,10 +2156,7 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
   for (int j = 0; j < mmax && !B.done; ++j) {
     T* Qj = B.V.p + (size_t)j * s;
     size_t hf = tm.begin("filter");
-    // Q_j is re-read by every step of the recurrence: a contiguous copy
-    // streams faster than the block's strided rows inside the basis
-    copy_block<T, T>(cx.stream, n, B.r_cur, Qj, B.ldV, Qc.p, lds);
-    apply_filter<T>(cx, op, B.r_cur, Qc.p, lds, coeffs, smin, smax, W.p, lds, b0.p, b1.p, lds,
+    apply_filter<T>(cx, op, B.r_cur, Qj, B.ldV, coeffs, smin, smax, W.p, lds, b0.p, b1.p, lds,
                     pbuf.p);
     tm.end(hf);
     if (j == 0) {
