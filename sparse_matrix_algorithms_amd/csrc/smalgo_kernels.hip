@@ -556,14 +556,21 @@ using mfma_t = typename std::conditional<std::is_same<TA, double>::value ||
                                              std::is_same<TC, double>::value,
                                          double, float>::type;
 
-constexpr int GKT = 16;  // K depth staged per LDS round
+// K depth staged per LDS round: 32 for FP32 tiles, 16 for FP64 ones, so that
+// both take ~17 KB of LDS per workgroup (occupancy); GPT = elements of one
+// 64-wide operand tile per thread.
+template <typename TM> struct GemmDepth { static constexpr int K = sizeof(TM) == 4 ? 32 : 16; };
 
 // C_part[z][i][j] = sum_{k in chunk z} A[k][i] * B[k][j]
+// The next K tile is loaded into registers while the current one is in the
+// MFMAs (software pipelining), so the global loads overlap the arithmetic.
 template <typename TA, typename TB, typename TM>
 __global__ __launch_bounds__(256) void gemm_tn_kernel(int n, int m, int b, int kchunk,
                                                       const TA* __restrict__ A, int lda,
                                                       const TB* __restrict__ B, int ldb,
                                                       TM* __restrict__ part) {
+  constexpr int GKT = GemmDepth<TM>::K;
+  constexpr int GPT = GKT * 64 / 256;
   __shared__ TM As[GKT][64 + 4];
   __shared__ TM Bs[GKT][64 + 4];
   using acc_t = typename Mfma<TM>::acc_t;
@@ -577,15 +584,27 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(int n, int m, int b, int k
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int c = 0; c < 2; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
-  for (int k0 = kb; k0 < ke; k0 += GKT) {
+  TM ra[GPT], rb[GPT];
+  auto load = [&](int k0) {
 #pragma unroll
-    for (int e = tid; e < GKT * 64; e += 256) {
+    for (int q = 0; q < GPT; ++q) {
+      const int e = tid + 256 * q;
       const int kk = e >> 6, ii = e & 63;
       const int k = k0 + kk;
-      As[kk][ii] = (k < ke && i0 + ii < m) ? (TM)A[(size_t)k * lda + i0 + ii] : TM(0);
-      Bs[kk][ii] = (k < ke && j0 + ii < b) ? (TM)B[(size_t)k * ldb + j0 + ii] : TM(0);
+      ra[q] = (k < ke && i0 + ii < m) ? (TM)A[(size_t)k * lda + i0 + ii] : TM(0);
+      rb[q] = (k < ke && j0 + ii < b) ? (TM)B[(size_t)k * ldb + j0 + ii] : TM(0);
+    }
+  };
+  if (kb < ke) load(kb);
+  for (int k0 = kb; k0 < ke; k0 += GKT) {
+#pragma unroll
+    for (int q = 0; q < GPT; ++q) {
+      const int e = tid + 256 * q;
+      As[e >> 6][e & 63] = ra[q];
+      Bs[e >> 6][e & 63] = rb[q];
     }
     __syncthreads();
+    if (k0 + GKT < ke) load(k0 + GKT);
 #pragma unroll
     for (int kk = 0; kk < GKT; kk += 4) {
       const int kr = kk + (lane >> 4), cl = lane & 15;
@@ -665,6 +684,7 @@ void gemm_tn(hipStream_t st, int n, int m, int b, double alpha, const TA* A, int
   const size_t need = (size_t)nsplit * m * b;
   if (need > work_elems) throw Error(-4, "gemm_tn workspace too small");
   int kchunk = (n + nsplit - 1) / nsplit;
+  constexpr int GKT = GemmDepth<TM>::K;
   kchunk = (kchunk + GKT - 1) / GKT * GKT;
   TM* part = reinterpret_cast<TM*>(work);
   if (n > 0) {
@@ -690,6 +710,8 @@ __global__ __launch_bounds__(256) void gemm_nn_kernel(int n, int m, int b, doubl
                                                       const TA* __restrict__ A, int lda,
                                                       const TB* __restrict__ B, int ldb,
                                                       double beta, TC* C, int ldc) {
+  constexpr int GKT = GemmDepth<TM>::K;
+  constexpr int GPT = GKT * 64 / 256;
   __shared__ TM As[64][GKT + 1];
   __shared__ TM Bs[GKT][64 + 4];
   using acc_t = typename Mfma<TM>::acc_t;
@@ -701,22 +723,32 @@ __global__ __launch_bounds__(256) void gemm_nn_kernel(int n, int m, int b, doubl
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int c = 0; c < 2; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
-  for (int k0 = 0; k0 < m; k0 += GKT) {
+  TM ra[GPT], rb[GPT];
+  // the next K tile goes to registers while the current one is in the MFMAs
+  auto load = [&](int k0) {
 #pragma unroll
-    for (int e = tid; e < 64 * GKT; e += 256) {
+    for (int q = 0; q < GPT; ++q) {
+      const int e = tid + 256 * q;
       const int rr = e / GKT, kk = e % GKT;
       const int r = r0 + rr, k = k0 + kk;
-      As[rr][kk] = (r < n && k < m) ? (TM)A[(size_t)r * lda + k] : TM(0);
-    }
-#pragma unroll
-    for (int e = tid; e < GKT * 64; e += 256) {
-      const int kk = e >> 6, cc = e & 63;
-      const int k = k0 + kk, c = c0 + cc;
+      ra[q] = (r < n && k < m) ? (TM)A[(size_t)r * lda + k] : TM(0);
+      const int kb = e >> 6, cc = e & 63;
+      const int k2 = k0 + kb, c = c0 + cc;
       TM v = TM(0);
-      if (k < m && c < b) v = TRANSB ? (TM)B[(size_t)c * ldb + k] : (TM)B[(size_t)k * ldb + c];
-      Bs[kk][cc] = v;
+      if (k2 < m && c < b) v = TRANSB ? (TM)B[(size_t)c * ldb + k2] : (TM)B[(size_t)k2 * ldb + c];
+      rb[q] = v;
+    }
+  };
+  if (m > 0) load(0);
+  for (int k0 = 0; k0 < m; k0 += GKT) {
+#pragma unroll
+    for (int q = 0; q < GPT; ++q) {
+      const int e = tid + 256 * q;
+      As[e / GKT][e % GKT] = ra[q];
+      Bs[e >> 6][e & 63] = rb[q];
     }
     __syncthreads();
+    if (k0 + GKT < m) load(k0 + GKT);
 #pragma unroll
     for (int kk = 0; kk < GKT; kk += 4) {
       const int kr = kk + (lane >> 4), cl = lane & 15;
