@@ -132,6 +132,11 @@ void gather_cols_rm(hipStream_t st, int n, int k, const T* S, int lds, const int
 void add_diag(hipStream_t st, int b, double* C, int ldc, double v);
 // C = A·B for upper-triangular column-major b x b (ld b) matrices
 void tri_mul_upper(hipStream_t st, int b, const double* A, const double* B, double* C);
+// G = RᵀR (upper Cholesky, upper triangle of G read) and Ri = R⁻¹ in one
+// workgroup, b <= chol_inv_max_b(); info as potrf's; G must not alias R, Ri
+int chol_inv_max_b();
+void chol_inv_upper(hipStream_t st, int b, const double* G, int ldg, double* R, int ldr,
+                    double* Ri, int ldri, int* info);
 // out[0..1] = info[0..1], out[2..2+b) = diag(R)  (R column-major, ld b)
 void pack_qr_status(hipStream_t st, int b, const int* info, const double* R, double* out);
 // s x s block, column-major: rows < rn and columns < r of R (ld r), zero elsewhere

@@ -1149,6 +1149,113 @@ void tri_mul_upper(hipStream_t st, int b, const double* A, const double* B, doub
   SMALGO_HIP(hipGetLastError());
 }
 
+// Upper Cholesky factor and its inverse of a symmetric positive definite
+// b x b matrix in one workgroup: G = RᵀR (upper triangle of G read,
+// column-major) and Ri = R⁻¹, both written in full with zeros below the
+// diagonal.  Thread (ty, tx) of a TG x TG grid holds rows 4ty..4ty+3 and
+// columns 4tx..4tx+3 of the matrix under elimination A and of Y = R⁻ᵀ.
+// Step k publishes row k of both, which is final by then, and updates the
+// rows below it:
+//   A_i· -= (A_ki / A_kk) A_k·,   Y_i· -= (A_ki / √A_kk) (Y_k· / √A_kk),
+// so that row k of R is A_k· / √A_kk and Y_k· / √A_kk is row k of R⁻ᵀ, i.e.
+// column k of R⁻¹.  Rows at or above k take a zero multiplier, so they keep
+// their final values; entries left of the diagonal of A are updated too but
+// never read again.  One barrier per step (the published rows are double
+// buffered); R and Ri are written from registers at the end.  info = k + 1
+// at the first pivot that is not positive (NaN included), as potrf reports
+// it, else 0; R and Ri are then not written.  G must not alias R or Ri.
+template <int TG>
+__global__ __launch_bounds__(TG * TG) void chol_inv_upper_kernel(
+    int b, const double* __restrict__ G, int ldg, double* __restrict__ R, int ldr,
+    double* __restrict__ Ri, int ldri, int* __restrict__ info) {
+  constexpr int NB = 4;
+  __shared__ double arow[2][NB * TG], yrow[2][NB * TG];
+  const int tid = threadIdx.x, ty = tid / TG, tx = tid % TG;
+  const int r0 = ty * NB, c0 = tx * NB;
+  double a[NB][NB], y[NB][NB], sk[NB];
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    sk[r] = 1.0;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const int i = r0 + r, j = c0 + c;
+      double v = i == j ? 1.0 : 0.0;
+      if (i < b && j < b) v = i <= j ? G[(size_t)j * ldg + i] : G[(size_t)i * ldg + j];
+      a[r][c] = v;
+      y[r][c] = i == j ? 1.0 : 0.0;
+    }
+  }
+  int bad = 0;
+  for (int kb = 0; kb * NB < b && bad == 0; ++kb) {
+#pragma unroll
+    for (int kr = 0; kr < NB; ++kr) {
+      const int k = kb * NB + kr;
+      if (k >= b) break;
+      const int buf = k & 1;
+      if (ty == kb) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          arow[buf][c0 + c] = a[kr][c];
+          yrow[buf][c0 + c] = y[kr][c];
+        }
+      }
+      __syncthreads();
+      const double piv = arow[buf][k];  // the same value in every thread
+      if (!(piv > 0.0)) {
+        bad = k + 1;
+        break;
+      }
+      const double s = rsqrt(piv), ip = s * s;
+      if (ty == kb) sk[kr] = s;
+      double ac[NB], yk[NB], ak[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        ac[c] = arow[buf][c0 + c];
+        yk[c] = yrow[buf][c0 + c] * s;
+        ak[c] = arow[buf][r0 + c];
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const bool below = r0 + r > k;
+        const double f = below ? ak[r] * ip : 0.0, l = below ? ak[r] * s : 0.0;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+          a[r][c] -= f * ac[c];
+          y[r][c] -= l * yk[c];
+        }
+      }
+    }
+  }
+  if (tid == 0) *info = bad;
+  if (bad != 0) return;
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    const int i = r0 + r;
+    if (i >= b) continue;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const int j = c0 + c;
+      if (j >= b) continue;
+      R[(size_t)j * ldr + i] = j >= i ? a[r][c] * sk[r] : 0.0;
+      Ri[(size_t)i * ldri + j] = y[r][c] * sk[r];  // (R⁻¹)_ji = Y_ij, zero for j > i
+    }
+  }
+}
+
+int chol_inv_max_b() { return 128; }
+
+void chol_inv_upper(hipStream_t st, int b, const double* G, int ldg, double* R, int ldr,
+                    double* Ri, int ldri, int* info) {
+  if (b < 1 || b > chol_inv_max_b()) throw Error(-1, "chol_inv_upper: b out of range");
+  if (b <= 64)
+    hipLaunchKernelGGL(chol_inv_upper_kernel<16>, dim3(1), dim3(256), 0, st, b, G, ldg, R, ldr,
+                       Ri, ldri, info);
+  else
+    hipLaunchKernelGGL(chol_inv_upper_kernel<32>, dim3(1), dim3(1024), 0, st, b, G, ldg, R, ldr,
+                       Ri, ldri, info);
+  SMALGO_HIP(hipGetLastError());
+}
+
 // out = [info[0], info[1], R(0,0), R(1,1), ..., R(b-1,b-1)] (R column-major, ld b)
 __global__ void pack_qr_status_kernel(int b, const int* __restrict__ info,
                                       const double* __restrict__ R, double* __restrict__ out) {

@@ -592,10 +592,36 @@ void apply_filter_noalloc(Ctx& cx, const Op& op, int b, const double* Q, int ldq
 // the caller keeps the first r (the reference's rule, polfed.jl:679-684).
 // ---------------------------------------------------------------------------
 struct OrthWork {
-  DBuf<double> G, Rinv, gemm_work, cn, red, R1, status;
+  DBuf<double> G, Rinv, gemm_work, cn, red, R1, R2, status;
   DBuf<int> info2;
   std::vector<double> hG, hstatus;
 };
+
+// R (upper, zeros below the diagonal) into ow.R2 and R⁻¹ into ow.Rinv for
+// the b x b Gram matrix in ow.G, which stays unchanged on the one-workgroup
+// path (b <= chol_inv_max_b()) and is overwritten by rocSOLVER's potrf
+// otherwise.  Returns potrf's info; R2 and Rinv are valid only for info 0.
+static int chol_inv_sync(Ctx& cx, Dense& dn, OrthWork& ow, int b) {
+  ow.R2.ensure((size_t)b * b);
+  ow.Rinv.ensure((size_t)b * b);
+  ow.info2.ensure(2);
+  if (b <= chol_inv_max_b()) {
+    chol_inv_upper(cx.stream, b, ow.G.p, b, ow.R2.p, b, ow.Rinv.p, b, ow.info2.p);
+    int h = 0;
+    SMALGO_HIP(hipMemcpyAsync(&h, ow.info2.p, sizeof(int), hipMemcpyDeviceToHost, cx.stream));
+    SMALGO_HIP(hipStreamSynchronize(cx.stream));
+    return h;
+  }
+  const int info = dn.potrf_upper(b, ow.G.p, b);
+  if (info != 0) return info;
+  zero_lower_cm(cx.stream, b, ow.G.p, b);
+  SMALGO_HIP(hipMemcpyAsync(ow.R2.p, ow.G.p, sizeof(double) * b * b, hipMemcpyDeviceToDevice,
+                            cx.stream));
+  SMALGO_HIP(hipMemcpyAsync(ow.Rinv.p, ow.G.p, sizeof(double) * b * b, hipMemcpyDeviceToDevice,
+                            cx.stream));
+  dn.trtri_upper(b, ow.Rinv.p, b);
+  return 0;
+}
 
 template <typename T>
 static int cholqr(Ctx& cx, Dense& dn, OrthWork& ow, int n, int b, const T* W, int ldw, T* Qout,
@@ -647,7 +673,7 @@ static int cholqr(Ctx& cx, Dense& dn, OrthWork& ow, int n, int b, const T* W, in
                           ow.gemm_work.p, ow.gemm_work.n);
     SMALGO_HIP(hipMemcpyAsync(ow.hG.data(), ow.G.p, sizeof(double) * b * b, hipMemcpyDeviceToHost,
                               cx.stream));
-    int info = dn.potrf_upper(b, ow.G.p, b);
+    int info = chol_inv_sync(cx, dn, ow, b);
     double tr = 0.0;
     for (int i = 0; i < b; ++i) tr += std::max(0.0, ow.hG[(size_t)i * b + i]);
     double shift = 11.0 * ((double)n * b + (double)b * (b + 1)) * u * tr;
@@ -661,16 +687,14 @@ static int cholqr(Ctx& cx, Dense& dn, OrthWork& ow, int n, int b, const T* W, in
       SMALGO_HIP(hipMemcpyAsync(ow.G.p, ow.hG.data(), sizeof(double) * b * b,
                                 hipMemcpyHostToDevice, cx.stream));
       add_diag(cx.stream, b, ow.G.p, b, shift);
-      info = dn.potrf_upper(b, ow.G.p, b);
+      info = chol_inv_sync(cx, dn, ow, b);
       shift *= 100.0;
     }
     if (shifted && passes == 2) passes = 3;
-    zero_lower_cm(cx.stream, b, ow.G.p, b);
-    SMALGO_HIP(hipMemcpyAsync(Rp.data(), ow.G.p, sizeof(double) * b * b, hipMemcpyDeviceToHost,
+    SMALGO_HIP(hipMemcpyAsync(Rp.data(), ow.R2.p, sizeof(double) * b * b, hipMemcpyDeviceToHost,
                               cx.stream));
-    dn.trtri_upper(b, ow.G.p, b);
     // dst = cur · R^{-1}; R^{-1} column-major -> transB
-    gemm_nn<T, double, T>(cx.stream, n, b, b, 1.0, cur, ldc, ow.G.p, b, true, 0.0, dst, ldd);
+    gemm_nn<T, double, T>(cx.stream, n, b, b, 1.0, cur, ldc, ow.Rinv.p, b, true, 0.0, dst, ldd);
     SMALGO_HIP(hipStreamSynchronize(cx.stream));
     if (full) {
       // Racc <- Rp · Racc   (both upper triangular, column-major)
@@ -712,23 +736,40 @@ static int cholqr_fast(Ctx& cx, Dense& dn, OrthWork& ow, int n, int b, const T* 
   ow.info2.ensure(2);
   ow.status.ensure((size_t)b + 2);
   ow.hstatus.resize((size_t)b + 2);
-  // pass 1: G = WᵀW = R_1ᵀR_1, tmp = W R_1⁻¹
-  gemm_tn<T, T, double>(cx.stream, n, b, b, 1.0, W, ldw, W, ldw, 0.0, ow.G.p, b, ow.gemm_work.p,
-                        ow.gemm_work.n);
-  dn.potrf_upper_async(b, ow.G.p, b, ow.info2.p);
-  zero_lower_cm(cx.stream, b, ow.G.p, b);
-  SMALGO_HIP(hipMemcpyAsync(ow.R1.p, ow.G.p, sizeof(double) * b * b, hipMemcpyDeviceToDevice,
-                            cx.stream));
-  dn.trtri_upper(b, ow.G.p, b);
-  gemm_nn<T, double, T>(cx.stream, n, b, b, 1.0, W, ldw, ow.G.p, b, true, 0.0, tmp, ldt);
-  // pass 2: tmpᵀtmp = R_2ᵀR_2, Q = tmp R_2⁻¹, R = R_2 R_1
-  gemm_tn<T, T, double>(cx.stream, n, b, b, 1.0, tmp, ldt, tmp, ldt, 0.0, ow.G.p, b,
-                        ow.gemm_work.p, ow.gemm_work.n);
-  dn.potrf_upper_async(b, ow.G.p, b, ow.info2.p + 1);
-  zero_lower_cm(cx.stream, b, ow.G.p, b);
-  tri_mul_upper(cx.stream, b, ow.G.p, ow.R1.p, Rdev);
-  dn.trtri_upper(b, ow.G.p, b);
-  gemm_nn<T, double, T>(cx.stream, n, b, b, 1.0, tmp, ldt, ow.G.p, b, true, 0.0, Qout, ldq);
+  if (b <= chol_inv_max_b()) {
+    // one-workgroup Cholesky + triangular inverse per pass (R_1 and R_2 kept)
+    ow.R2.ensure((size_t)b * b);
+    ow.Rinv.ensure((size_t)b * b);
+    // pass 1: G = WᵀW = R_1ᵀR_1, tmp = W R_1⁻¹
+    gemm_tn<T, T, double>(cx.stream, n, b, b, 1.0, W, ldw, W, ldw, 0.0, ow.G.p, b,
+                          ow.gemm_work.p, ow.gemm_work.n);
+    chol_inv_upper(cx.stream, b, ow.G.p, b, ow.R1.p, b, ow.Rinv.p, b, ow.info2.p);
+    gemm_nn<T, double, T>(cx.stream, n, b, b, 1.0, W, ldw, ow.Rinv.p, b, true, 0.0, tmp, ldt);
+    // pass 2: tmpᵀtmp = R_2ᵀR_2, Q = tmp R_2⁻¹, R = R_2 R_1
+    gemm_tn<T, T, double>(cx.stream, n, b, b, 1.0, tmp, ldt, tmp, ldt, 0.0, ow.G.p, b,
+                          ow.gemm_work.p, ow.gemm_work.n);
+    chol_inv_upper(cx.stream, b, ow.G.p, b, ow.R2.p, b, ow.Rinv.p, b, ow.info2.p + 1);
+    tri_mul_upper(cx.stream, b, ow.R2.p, ow.R1.p, Rdev);
+    gemm_nn<T, double, T>(cx.stream, n, b, b, 1.0, tmp, ldt, ow.Rinv.p, b, true, 0.0, Qout, ldq);
+  } else {
+    // pass 1: G = WᵀW = R_1ᵀR_1, tmp = W R_1⁻¹
+    gemm_tn<T, T, double>(cx.stream, n, b, b, 1.0, W, ldw, W, ldw, 0.0, ow.G.p, b,
+                          ow.gemm_work.p, ow.gemm_work.n);
+    dn.potrf_upper_async(b, ow.G.p, b, ow.info2.p);
+    zero_lower_cm(cx.stream, b, ow.G.p, b);
+    SMALGO_HIP(hipMemcpyAsync(ow.R1.p, ow.G.p, sizeof(double) * b * b, hipMemcpyDeviceToDevice,
+                              cx.stream));
+    dn.trtri_upper(b, ow.G.p, b);
+    gemm_nn<T, double, T>(cx.stream, n, b, b, 1.0, W, ldw, ow.G.p, b, true, 0.0, tmp, ldt);
+    // pass 2: tmpᵀtmp = R_2ᵀR_2, Q = tmp R_2⁻¹, R = R_2 R_1
+    gemm_tn<T, T, double>(cx.stream, n, b, b, 1.0, tmp, ldt, tmp, ldt, 0.0, ow.G.p, b,
+                          ow.gemm_work.p, ow.gemm_work.n);
+    dn.potrf_upper_async(b, ow.G.p, b, ow.info2.p + 1);
+    zero_lower_cm(cx.stream, b, ow.G.p, b);
+    tri_mul_upper(cx.stream, b, ow.G.p, ow.R1.p, Rdev);
+    dn.trtri_upper(b, ow.G.p, b);
+    gemm_nn<T, double, T>(cx.stream, n, b, b, 1.0, tmp, ldt, ow.G.p, b, true, 0.0, Qout, ldq);
+  }
   pack_qr_status(cx.stream, b, ow.info2.p, Rdev, ow.status.p);
   SMALGO_HIP(hipMemcpyAsync(ow.hstatus.data(), ow.status.p, sizeof(double) * (b + 2),
                             hipMemcpyDeviceToHost, cx.stream));
@@ -1157,8 +1198,9 @@ struct SlicePack {
 };
 
 // Packing.  Slices are taken in order of decreasing degree.  With a fixed
-// width every panel is wfix wide and a slice that does not fit the room left
-// continues in the next panel.  Otherwise panels have their own widths (up
+// width every panel is wfix wide, except that a last panel the columns do not
+// fill is cut to them (rounded up to 4); a slice that does not fit the room
+// left continues in the next panel.  Otherwise panels have their own widths (up
 // to wmax, multiples of 4): a slice joins the open panel when the measured
 // cost of widening it, K_p (t(w+s) - t(w)), is below the cost of a panel of
 // its own, K_i t(s), where t(w) is the tuned time of one step at width w.
@@ -1215,7 +1257,7 @@ static SlicePack pack_slices(const std::vector<int>& widths, const std::vector<i
   }
   for (size_t p = 0; p < pk.pan.size(); ++p) {
     SlicePack::Panel& P = pk.pan[p];
-    P.w = wfix > 0 ? wfix : round_up((int)pk.col_slice[p].size(), 4);
+    P.w = round_up((int)pk.col_slice[p].size(), 4);  // wfix for every full panel
     pk.col_slice[p].resize(P.w, -1);
     pk.col_idx[p].resize(P.w, 0);
     pk.steps += P.K;
@@ -1275,7 +1317,7 @@ struct SliceFilter {
   }
 
   // Build the packing, the per-column coefficient table and the step list.
-  // wfix > 0 gives every panel that width; otherwise panels are sized by the
+  // wfix > 0 gives every full panel that width; otherwise panels are sized by the
   // measured step time of each width, up to the cache-aware panel width.
   void plan(const std::vector<int>& w, const std::vector<SliceCoeffs>& co, double smin,
             double smax, int wfix) {
