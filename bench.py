@@ -118,13 +118,6 @@ def valid_pairs(lam, res, emin, emax, anorm):
     return int(ok.sum()), ok
 
 
-def under_converged(lam, res, emin, emax, anorm):
-    """A returned pair inside the interval with a residual between the 1e-10
-    bound and 1e-6*|A|: a real eigenpair the filter did not finish."""
-    inside = (lam >= emin) & (lam <= emax)
-    return bool(np.any(inside & (res > 1e-10 * anorm) & (res < 1e-6 * anorm)))
-
-
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -485,24 +478,14 @@ def bench_slices(args):
     ns = args.slices
     model = None
     if args.slice_learned:
-        from sparse_matrix_algorithms_amd.learned import MLP, choose_parameters
+        from sparse_matrix_algorithms_amd.learned import MLP
         model = MLP.load(ctx=ctx)
-    slices = []
+    base = []
     for i in range(ns):
         c = -4.0 + 8.0 * (i + 0.5) / ns
         nev = SLICE_NEV[i % len(SLICE_NEV)]
         a, b = kpm.interval_for_count(mu, n, args.fill * nev, smin, smax, c)
-        s = slice_width(nev)
-        src = "plain"
-        if model is not None:
-            K, mm, src = choose_parameters(model, mu[:64], n, nev, a, b, smin, smax, s,
-                                           precision=prec)
-        else:
-            K, mm = kpm.plain_parameters(n, nev, a, b, smin, smax, s, args.kappa,
-                                         args.mmax_factor, precision=prec)
-        slices.append(dict(emin=a, emax=b, sigma=c, nev=nev, s=s, degree=K, mmax=mm))
-        log(f"[bench] slice {i}: centre {c:+.3f} [{a:.5f},{b:.5f}] nev={nev} s={s} K={K} "
-            f"mmax={mm} ({src})")
+        base.append(dict(emin=a, emax=b, sigma=c, nev=nev, s=slice_width(nev)))
     kw = dict(smin=smin, smax=smax, tol=1e-10 * anorm, precision=prec, ctx=ctx, want_X=False,
               refine_guard=args.refine_guard, refine_degree=args.refine_degree,
               refine_maxiter=args.refine_maxiter,
@@ -520,33 +503,36 @@ def bench_slices(args):
             per.append(valid_pairs(lam, res, d["emin"], d["emax"], anorm)[0])
         return per
 
-    outs, dt = batch(slices)
-    st = outs[0][3]
-    # Learned parameters can under-shoot a slice's degree.  Such a slice returns
-    # real eigenpairs inside its interval whose residuals sit just above tol
-    # (spurious Ritz values in the interval sit near 1e-3*|A|).  Those slices are
-    # solved again, in one more batch, with the plain rule; the time counts.
     retry = None
-    if model is not None and not args.no_retry:
-        redo = [i for i, (d, o) in enumerate(zip(slices, outs))
-                if under_converged(o[0], o[2], d["emin"], d["emax"], anorm)]
-        if redo:
-            plain = []
-            for i in redo:
-                d = slices[i]
-                K, mm = kpm.plain_parameters(n, d["nev"], d["emin"], d["emax"], smin, smax,
-                                             d["s"], args.kappa, args.mmax_factor,
-                                             precision=prec)
-                plain.append(dict(d, degree=K, mmax=mm))
-                log(f"[bench] slice {i} under-converged: again with the plain rule K={K} "
-                    f"mmax={mm}")
-            outs2, dt2 = batch(plain)
-            before = count_valid([slices[i] for i in redo], [outs[i] for i in redo])
-            for j, i in enumerate(redo):
-                outs[i], slices[i] = outs2[j], plain[j]
-            dt += dt2
-            retry = dict(slices=redo, seconds=round(dt2, 3), valid_before=before,
-                         valid_after=count_valid(plain, outs2))
+    if model is not None:
+        # Learned parameters, and the plain rule again for every slice whose learned
+        # degree fell short (learned.polfed_slices_learned); the retry's time counts.
+        from sparse_matrix_algorithms_amd.learned import polfed_slices_learned
+        lkw = {k: v for k, v in kw.items() if k not in ("smin", "smax", "tol", "precision")}
+        with stdout_to_stderr():
+            outs, slices, info = polfed_slices_learned(
+                dm, base, mu, model, smin, smax, precision=prec, retry=not args.no_retry,
+                use_graph=not args.eager, show_progress=True, **lkw)
+        dt = sum(info["seconds"])
+        st = info["first"].get(0, outs[0])[3]  # batch-level stats of the first batch
+        if info["retried"]:
+            redo = info["retried"]
+            retry = dict(slices=redo, seconds=round(info["seconds"][1], 3),
+                         valid_before=count_valid([slices[i] for i in redo],
+                                                  [info["first"][i] for i in redo]),
+                         valid_after=count_valid([slices[i] for i in redo],
+                                                 [outs[i] for i in redo]))
+    else:
+        slices = []
+        for d in base:
+            K, mm = kpm.plain_parameters(n, d["nev"], d["emin"], d["emax"], smin, smax, d["s"],
+                                         args.kappa, args.mmax_factor, precision=prec)
+            slices.append(dict(d, degree=K, mmax=mm, source="plain"))
+        outs, dt = batch(slices)
+        st = outs[0][3]
+    for i, d in enumerate(slices):
+        log(f"[bench] slice {i}: centre {d['sigma']:+.3f} [{d['emin']:.5f},{d['emax']:.5f}] "
+            f"nev={d['nev']} s={d['s']} K={d['degree']} mmax={d['mmax']} ({d['source']})")
     per = count_valid(slices, outs)
     total = sum(per)
     log(f"[bench] batch of {ns}: {dt:.2f}s, {total} valid pairs, t_filter={st['t_filter']:.2f}s "

@@ -184,3 +184,74 @@ def choose_parameters(model: Optional[MLP], mu: np.ndarray, n: int, nev: int, em
         K = max(1, math.ceil(MIXED_DEGREE_SCALE * K))
         mmax = max(1, math.ceil(MIXED_MMAX_SCALE * mmax - 1e-9))
     return K, mmax, "learned"
+
+
+def under_converged(lam, res, emin: float, emax: float, anorm: float,
+                    tol_rel: float = 1e-10) -> bool:
+    """True when a returned pair inside [emin, emax] has a residual between
+    tol_rel·‖A‖ and 1e-6·‖A‖: a real eigenpair the filter did not finish.
+    Spurious Ritz values inside an interval sit near 1e-3·‖A‖ and do not
+    count."""
+    lam = np.asarray(lam)
+    res = np.asarray(res)
+    inside = (lam >= emin) & (lam <= emax)
+    return bool(np.any(inside & (res > tol_rel * anorm) & (res < 1e-6 * anorm)))
+
+
+def polfed_slices_learned(A, slices, mu: np.ndarray, model: Optional[MLP], smin: float,
+                          smax: float, precision: str = "fp32", tol_rel: float = 1e-10,
+                          retry: bool = True, **kw):
+    """Batched POLFED slices (``solvers.polfed_slices``) with learned filter
+    parameters, and the plain rule for every slice whose learned degree fell
+    short.
+
+    ``slices``: dicts with ``emin, emax, nev, s`` and optional ``sigma``;
+    ``mu``: KPM moments of ``A`` on [smin, smax]; ``kw`` goes to
+    ``polfed_slices``.  The learned answer can under-shoot a slice's degree
+    where the matrix lies outside the training range; such a slice returns
+    real eigenpairs inside its interval whose residuals sit above the
+    tolerance (``under_converged``).  With ``retry`` those slices are solved
+    again, in one more batch, with the plain rule.  Without a model every
+    slice takes the plain rule and nothing is retried.
+
+    Returns ``(outs, params, info)``: one ``(λ, X, res, stats)`` per slice,
+    the parameters each answer was computed with (``source``: "learned",
+    "fallback", "plain" or "plain retry"), and ``info`` with the retried
+    indices (``retried``), their first answers (``first``) and the wall time of
+    each batch (``seconds``).
+    """
+    import time
+
+    from .solvers import polfed_slices
+    n = A.shape[0] if hasattr(A, "shape") else A.n
+    anorm = max(abs(smin), abs(smax))
+
+    def run(ps):
+        t = time.perf_counter()
+        o = polfed_slices(A, [{k: v for k, v in p.items() if k != "source"} for p in ps],
+                          smin=smin, smax=smax, tol=tol_rel * anorm, precision=precision, **kw)
+        return o, time.perf_counter() - t
+
+    params = []
+    for d in slices:
+        K, mm, src = choose_parameters(model, mu[:NMOM], n, d["nev"], d["emin"], d["emax"],
+                                       smin, smax, d["s"], precision=precision)
+        params.append(dict(d, degree=K, mmax=mm, source=src))
+    outs, dt = run(params)
+    info = dict(retried=[], first={}, seconds=[dt])
+    if retry and model is not None:
+        info["retried"] = [i for i, (d, o) in enumerate(zip(params, outs))
+                           if under_converged(o[0], o[2], d["emin"], d["emax"], anorm, tol_rel)]
+    if info["retried"]:
+        plain = []
+        for i in info["retried"]:
+            d = params[i]
+            K, mm = plain_parameters(n, d["nev"], d["emin"], d["emax"], smin, smax, d["s"],
+                                     precision=precision)
+            plain.append(dict(d, degree=K, mmax=mm, source="plain retry"))
+        outs2, dt2 = run(plain)
+        info["seconds"].append(dt2)
+        for j, i in enumerate(info["retried"]):
+            info["first"][i] = outs[i]
+            outs[i], params[i] = outs2[j], plain[j]
+    return outs, params, info

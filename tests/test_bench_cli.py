@@ -1,5 +1,6 @@
 """bench.py's host-side rules, on the CPU: the valid-pair count, the
-under-converged slice test, and the one-GPU guard."""
+under-converged slice test of the learned slice driver
+(learned.under_converged), and the one-GPU guard."""
 import importlib.util
 import subprocess
 import sys
@@ -18,6 +19,7 @@ def _bench():
 
 
 def test_valid_pairs_and_under_converged():
+    from sparse_matrix_algorithms_amd.learned import under_converged
     b = _bench()
     anorm = 10.0
     lam = np.array([-0.2, 0.0, 0.1, 0.3, 0.5])
@@ -25,13 +27,13 @@ def test_valid_pairs_and_under_converged():
     n, ok = b.valid_pairs(lam, res, -0.25, 0.35, anorm)
     assert n == 3 and ok.tolist() == [True, True, False, True, False]
     # a spurious Ritz value (residual ~1e-3·|A|) alone is not under-convergence
-    assert not b.under_converged(lam, res, -0.25, 0.35, anorm)
+    assert not under_converged(lam, res, -0.25, 0.35, anorm)
     # a real pair just above the 1e-10·|A| bound is
     res2 = res.copy()
     res2[1] = 3e-9
-    assert b.under_converged(lam, res2, -0.25, 0.35, anorm)
+    assert under_converged(lam, res2, -0.25, 0.35, anorm)
     # outside the interval it does not count
-    assert not b.under_converged(lam, res2, 0.05, 0.35, anorm)
+    assert not under_converged(lam, res2, 0.05, 0.35, anorm)
 
 
 def test_more_than_one_gpu_is_refused():

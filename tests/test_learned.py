@@ -136,3 +136,4 @@ def test_mixed_scales_chosen_on_training_records():
     assert set(keys) <= set(tr) and all(tr[k]["records"] == 256 for k in keys)
     best = max(keys, key=lambda k: (tr[k]["success"], tr[k]["eigenpairs_per_s"]))
     assert best == f"mixed/{learned.MIXED_DEGREE_SCALE}/{learned.MIXED_MMAX_SCALE}"
+
