@@ -558,18 +558,25 @@ using mfma_t = typename std::conditional<std::is_same<TA, double>::value ||
 
 // K depth staged per LDS round: 32 for FP32 tiles, 16 for FP64 ones, so that
 // both take ~17 KB of LDS per workgroup (occupancy); GPT = elements of one
-// 64-wide operand tile per thread.
-template <typename TM> struct GemmDepth { static constexpr int K = sizeof(TM) == 4 ? 32 : 16; };
+// 64-wide operand tile per thread.  D = K tiles whose global loads are in
+// flight at once (a ring of registers, raw operand types): 4 for FP64 tiles,
+// whose K = 64 products then issue every load before the first MFMA, 1 for
+// FP32 ones (twice the elements per tile; a deeper ring halves their
+// occupancy).
+template <typename TM> struct GemmDepth {
+  static constexpr int K = sizeof(TM) == 4 ? 32 : 16;
+  static constexpr int D = sizeof(TM) == 4 ? 1 : 4;
+};
 
 // C_part[z][i][j] = sum_{k in chunk z} A[k][i] * B[k][j]
-// The next K tile is loaded into registers while the current one is in the
-// MFMAs (software pipelining), so the global loads overlap the arithmetic.
+// Tile t + D is loaded into the ring slot of tile t while tile t is in the
+// MFMAs, so D tiles of global loads overlap the arithmetic.
 template <typename TA, typename TB, typename TM>
 __global__ __launch_bounds__(256) void gemm_tn_kernel(int n, int m, int b, int kchunk,
                                                       const TA* __restrict__ A, int lda,
                                                       const TB* __restrict__ B, int ldb,
                                                       TM* __restrict__ part) {
-  constexpr int GKT = GemmDepth<TM>::K;
+  constexpr int GKT = GemmDepth<TM>::K, D = GemmDepth<TM>::D;
   constexpr int GPT = GKT * 64 / 256;
   __shared__ TM As[GKT][64 + 4];
   __shared__ TM Bs[GKT][64 + 4];
@@ -584,38 +591,47 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(int n, int m, int b, int k
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int c = 0; c < 2; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
-  TM ra[GPT], rb[GPT];
-  auto load = [&](int k0) {
+  TA ra[D][GPT];
+  TB rb[D][GPT];
+  auto load = [&](TA* xa, TB* xb, int k0) {
 #pragma unroll
     for (int q = 0; q < GPT; ++q) {
       const int e = tid + 256 * q;
       const int kk = e >> 6, ii = e & 63;
       const int k = k0 + kk;
-      ra[q] = (k < ke && i0 + ii < m) ? (TM)A[(size_t)k * lda + i0 + ii] : TM(0);
-      rb[q] = (k < ke && j0 + ii < b) ? (TM)B[(size_t)k * ldb + j0 + ii] : TM(0);
+      xa[q] = (k < ke && i0 + ii < m) ? A[(size_t)k * lda + i0 + ii] : TA(0);
+      xb[q] = (k < ke && j0 + ii < b) ? B[(size_t)k * ldb + j0 + ii] : TB(0);
     }
   };
-  if (kb < ke) load(kb);
-  for (int k0 = kb; k0 < ke; k0 += GKT) {
 #pragma unroll
-    for (int q = 0; q < GPT; ++q) {
-      const int e = tid + 256 * q;
-      As[e >> 6][e & 63] = ra[q];
-      Bs[e >> 6][e & 63] = rb[q];
-    }
-    __syncthreads();
-    if (k0 + GKT < ke) load(k0 + GKT);
+  for (int d = 0; d < D; ++d)
+    if (kb + d * GKT < ke) load(ra[d], rb[d], kb + d * GKT);
+  for (int k0 = kb; k0 < ke; k0 += D * GKT) {
 #pragma unroll
-    for (int kk = 0; kk < GKT; kk += 4) {
-      const int kr = kk + (lane >> 4), cl = lane & 15;
-      const TM a0 = As[kr][wi + cl], a1 = As[kr][wi + 16 + cl];
-      const TM b0 = Bs[kr][wj + cl], b1 = Bs[kr][wj + 16 + cl];
-      acc[0][0] = Mfma<TM>::op(a0, b0, acc[0][0]);
-      acc[0][1] = Mfma<TM>::op(a0, b1, acc[0][1]);
-      acc[1][0] = Mfma<TM>::op(a1, b0, acc[1][0]);
-      acc[1][1] = Mfma<TM>::op(a1, b1, acc[1][1]);
+    for (int d = 0; d < D; ++d) {
+      const int kt = k0 + d * GKT;
+      if (kt < ke) {  // the same in every thread
+#pragma unroll
+        for (int q = 0; q < GPT; ++q) {
+          const int e = tid + 256 * q;
+          As[e >> 6][e & 63] = (TM)ra[d][q];
+          Bs[e >> 6][e & 63] = (TM)rb[d][q];
+        }
+        __syncthreads();
+        if (kt + D * GKT < ke) load(ra[d], rb[d], kt + D * GKT);
+#pragma unroll
+        for (int kk = 0; kk < GKT; kk += 4) {
+          const int kr = kk + (lane >> 4), cl = lane & 15;
+          const TM a0 = As[kr][wi + cl], a1 = As[kr][wi + 16 + cl];
+          const TM b0 = Bs[kr][wj + cl], b1 = Bs[kr][wj + 16 + cl];
+          acc[0][0] = Mfma<TM>::op(a0, b0, acc[0][0]);
+          acc[0][1] = Mfma<TM>::op(a0, b1, acc[0][1]);
+          acc[1][0] = Mfma<TM>::op(a1, b0, acc[1][0]);
+          acc[1][1] = Mfma<TM>::op(a1, b1, acc[1][1]);
+        }
+        __syncthreads();
+      }
     }
-    __syncthreads();
   }
   TM* P = part + (size_t)z * m * b;
 #pragma unroll
@@ -705,12 +721,15 @@ void gemm_tn(hipStream_t st, int n, int m, int b, double alpha, const TA* A, int
 }
 
 // C[i][j] = alpha * sum_k A[i][k] B(k, j) + beta * C[i][j]
+// K tiles go through the same register ring as in gemm_tn_kernel; with
+// FP64 tiles and beta != 0 the C elements are loaded before the K loop, so
+// their latency overlaps the arithmetic too.
 template <typename TA, typename TB, typename TC, typename TM, bool TRANSB>
 __global__ __launch_bounds__(256) void gemm_nn_kernel(int n, int m, int b, double alpha,
                                                       const TA* __restrict__ A, int lda,
                                                       const TB* __restrict__ B, int ldb,
                                                       double beta, TC* C, int ldc) {
-  constexpr int GKT = GemmDepth<TM>::K;
+  constexpr int GKT = GemmDepth<TM>::K, D = GemmDepth<TM>::D;
   constexpr int GPT = GKT * 64 / 256;
   __shared__ TM As[64][GKT + 1];
   __shared__ TM Bs[GKT][64 + 4];
@@ -723,43 +742,65 @@ __global__ __launch_bounds__(256) void gemm_nn_kernel(int n, int m, int b, doubl
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int c = 0; c < 2; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
-  TM ra[GPT], rb[GPT];
-  // the next K tile goes to registers while the current one is in the MFMAs
-  auto load = [&](int k0) {
+  constexpr bool early_c = D > 1;
+  TC cold[2][2][4];
+  if (early_c && beta != 0.0) {
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = r0 + wi + ti * 16 + Mfma<TM>::drow(lane, r);
+          const int j = c0 + wj + tj * 16 + (lane & 15);
+          cold[ti][tj][r] = (i < n && j < b) ? C[(size_t)i * ldc + j] : TC(0);
+        }
+  }
+  TA ra[D][GPT];
+  TB rb[D][GPT];
+  auto load = [&](TA* xa, TB* xb, int k0) {
 #pragma unroll
     for (int q = 0; q < GPT; ++q) {
       const int e = tid + 256 * q;
       const int rr = e / GKT, kk = e % GKT;
       const int r = r0 + rr, k = k0 + kk;
-      ra[q] = (r < n && k < m) ? (TM)A[(size_t)r * lda + k] : TM(0);
+      xa[q] = (r < n && k < m) ? A[(size_t)r * lda + k] : TA(0);
       const int kb = e >> 6, cc = e & 63;
       const int k2 = k0 + kb, c = c0 + cc;
-      TM v = TM(0);
-      if (k2 < m && c < b) v = TRANSB ? (TM)B[(size_t)c * ldb + k2] : (TM)B[(size_t)k2 * ldb + c];
-      rb[q] = v;
+      TB v = TB(0);
+      if (k2 < m && c < b) v = TRANSB ? B[(size_t)c * ldb + k2] : B[(size_t)k2 * ldb + c];
+      xb[q] = v;
     }
   };
-  if (m > 0) load(0);
-  for (int k0 = 0; k0 < m; k0 += GKT) {
 #pragma unroll
-    for (int q = 0; q < GPT; ++q) {
-      const int e = tid + 256 * q;
-      As[e / GKT][e % GKT] = ra[q];
-      Bs[e >> 6][e & 63] = rb[q];
-    }
-    __syncthreads();
-    if (k0 + GKT < m) load(k0 + GKT);
+  for (int d = 0; d < D; ++d)
+    if (d * GKT < m) load(ra[d], rb[d], d * GKT);
+  for (int k0 = 0; k0 < m; k0 += D * GKT) {
 #pragma unroll
-    for (int kk = 0; kk < GKT; kk += 4) {
-      const int kr = kk + (lane >> 4), cl = lane & 15;
-      const TM a0 = As[wi + cl][kr], a1 = As[wi + 16 + cl][kr];
-      const TM b0 = Bs[kr][wj + cl], b1 = Bs[kr][wj + 16 + cl];
-      acc[0][0] = Mfma<TM>::op(a0, b0, acc[0][0]);
-      acc[0][1] = Mfma<TM>::op(a0, b1, acc[0][1]);
-      acc[1][0] = Mfma<TM>::op(a1, b0, acc[1][0]);
-      acc[1][1] = Mfma<TM>::op(a1, b1, acc[1][1]);
+    for (int d = 0; d < D; ++d) {
+      const int kt = k0 + d * GKT;
+      if (kt < m) {
+#pragma unroll
+        for (int q = 0; q < GPT; ++q) {
+          const int e = tid + 256 * q;
+          As[e / GKT][e % GKT] = (TM)ra[d][q];
+          Bs[e >> 6][e & 63] = (TM)rb[d][q];
+        }
+        __syncthreads();
+        if (kt + D * GKT < m) load(ra[d], rb[d], kt + D * GKT);
+#pragma unroll
+        for (int kk = 0; kk < GKT; kk += 4) {
+          const int kr = kk + (lane >> 4), cl = lane & 15;
+          const TM a0 = As[wi + cl][kr], a1 = As[wi + 16 + cl][kr];
+          const TM b0 = Bs[kr][wj + cl], b1 = Bs[kr][wj + 16 + cl];
+          acc[0][0] = Mfma<TM>::op(a0, b0, acc[0][0]);
+          acc[0][1] = Mfma<TM>::op(a0, b1, acc[0][1]);
+          acc[1][0] = Mfma<TM>::op(a1, b0, acc[1][0]);
+          acc[1][1] = Mfma<TM>::op(a1, b1, acc[1][1]);
+        }
+        __syncthreads();
+      }
     }
-    __syncthreads();
   }
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
@@ -771,7 +812,8 @@ __global__ __launch_bounds__(256) void gemm_nn_kernel(int n, int m, int b, doubl
         const int j = c0 + wj + tj * 16 + (lane & 15);
         if (i < n && j < b) {
           double v = alpha * (double)acc[ti][tj][r];
-          if (beta != 0.0) v += beta * (double)C[(size_t)i * ldc + j];
+          if (beta != 0.0)
+            v += beta * (double)(early_c ? cold[ti][tj][r] : C[(size_t)i * ldc + j]);
           C[(size_t)i * ldc + j] = (TC)v;
         }
       }
