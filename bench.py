@@ -232,8 +232,7 @@ def main():
     bytes_per_launch = st["spmm_bytes"] / max(launches, 1)
     achieved = bytes_per_launch / avg_launch / 1e9
     tname = "float" if head_prec == "fp32" else "double"
-    kname = (f"{kernel_base(st)}<{tname},{int(st['tile_vec'])},{int(st['tile_lpr'])},"
-             f"{int(st['tile_rt'])},false>")
+    kname = kernel_name(st, tname)
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs of
     # the same kernel and tile on this workload (scaled by the copy-kernel calibration)
     traffic = None
@@ -310,10 +309,17 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def kernel_base(st):
-    """Name of the fused filter kernel the autotuner picked (the CSR tile staged
-    in LDS or read per lane)."""
-    return "clenshaw_lds_kernel" if st.get("tile_lds", 0) else "clenshaw_kernel"
+def kernel_name(st, tname, percol=False):
+    """Template name of the fused filter kernel the autotuner picked: the CSR
+    tile read per lane (clenshaw_kernel) or staged in LDS (clenshaw_lds_kernel,
+    whose last flag is the prefetch form: own-row loads before the staging
+    barrier, used by tiles of one or two rows per team)."""
+    lds, rt = int(st.get("tile_lds", 0)), int(st["tile_rt"])
+    args = (f"{tname},{int(st['tile_vec'])},{int(st['tile_lpr'])},{rt},"
+            f"{'true' if percol else 'false'}")
+    if lds == 0:
+        return f"clenshaw_kernel<{args}>"
+    return f"clenshaw_lds_kernel<{args},{'true' if lds == 2 and rt <= 2 else 'false'}>"
 
 
 def cpu_info():
@@ -430,8 +436,7 @@ def filter_roofline(st, prec, percol, bytes_key="spmm_bytes", launches=None):
     bpl = st[bytes_key] / max(launches, 1)
     achieved = bpl / avg_launch / 1e9
     tname = "float" if prec == "fp32" else "double"
-    kname = (f"{kernel_base(st)}<{tname},{int(st['tile_vec'])},{int(st['tile_lpr'])},"
-             f"{int(st['tile_rt'])},{'true' if percol else 'false'}>")
+    kname = kernel_name(st, tname, percol)
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=kname,
                 bytes_per_launch=bpl, avg_launch_us=round(avg_launch * 1e6, 3),

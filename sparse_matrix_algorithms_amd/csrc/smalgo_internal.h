@@ -55,7 +55,8 @@ struct FilterTile {
   int vec = 1;   // elements per lane per load
   int lpr = 64;  // lanes per row (rows per wavefront = 64/lpr)
   int rt = 1;    // consecutive rows per lane team
-  int lds = 0;   // 1: the workgroup's CSR tile is staged in LDS first
+  int lds = 0;   // 1: the workgroup's CSR tile is staged in LDS first; 2: staged, with the
+                 // own-row loads issued before the staging barrier (rt <= 2)
 };
 
 template <typename T>

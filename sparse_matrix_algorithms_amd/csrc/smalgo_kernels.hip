@@ -221,7 +221,12 @@ __global__ __launch_bounds__(256) void clenshaw_kernel(
 // them from LDS (broadcast reads) instead of each issuing its own global
 // index and value loads per nonzero.  Tiles with more than 12 nonzeros per
 // row on average take the global path.  Gather form only (X1 set, P null).
-template <typename T, int V, int LPR, int RT, bool PERCOL>
+//
+// PF (prefetch): each lane issues its own rows' Q, X1 and X2 loads before the
+// staging barrier and gathers 8/RT neighbour rows per loop trip, so a row of
+// up to eight nonzeros costs two memory round trips per wave (staging with
+// the own-row loads, then the whole gather) instead of four.
+template <typename T, int V, int LPR, int RT, bool PERCOL, bool PF>
 __global__ __launch_bounds__(256) void clenshaw_lds_kernel(
     int n, int b, int nrowtiles, int ntiles, int per_xcd, const int* __restrict__ rowptr,
     const int* __restrict__ colidx, const T* __restrict__ val, const T* __restrict__ Q, int ldq,
@@ -239,6 +244,26 @@ __global__ __launch_bounds__(256) void clenshaw_lds_kernel(
   const int rtile = tile - chunk * nrowtiles;
   const int rbase = rtile * RPB;
   const int nrows = min(RPB, n - rbase);  // >= 1: rtile < nrowtiles
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int row0 = rbase + (wave * RPW + lane / LPR) * RT;
+  const int cic = (lane % LPR) * V;
+  const int c = chunk * (LPR * V) + cic;
+  const bool active = row0 < n && c < b;  // lane owns at least one element
+  const bool full = (c + V <= b);
+  const long long coff = chunk_stride ? (long long)chunk * chunk_stride + cic : c;
+  T qv[PF ? RT : 1][V], xv[PF ? RT : 1][V], yv[PF ? RT : 1][V];
+  if constexpr (PF) {
+    if (active && full) {
+#pragma unroll
+      for (int r = 0; r < RT; ++r) {
+        const size_t row = (size_t)min(row0 + r, n - 1);  // clamped: unused past n
+        if (Q != nullptr) ldv<T, V>(Q + coff + row * ldq, qv[r]);
+        ldv<T, V>(X1 + coff + row * ld1, xv[r]);
+        if (X2 != nullptr) ldv_alias<T, V>(X2 + coff + row * ld2, yv[r]);
+      }
+    }
+  }
   const int e0 = rowptr[rbase];
   const int ne = rowptr[rbase + nrows] - e0;
   const bool staged = ne <= MAXE;  // uniform over the workgroup
@@ -250,19 +275,11 @@ __global__ __launch_bounds__(256) void clenshaw_lds_kernel(
     }
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int row0 = rbase + (wave * RPW + lane / LPR) * RT;
-  if (row0 >= n) return;
-  const int cic = (lane % LPR) * V;
-  const int c = chunk * (LPR * V) + cic;
-  if (c >= b) return;
-  const long long coff = chunk_stride ? (long long)chunk * chunk_stride + cic : c;
+  if (!active) return;
   if (Q) Q += coff;
   X1 += coff;
   if (X2) X2 += coff;
   out += coff;
-  const bool full = (c + V <= b);
   int p0[RT], len[RT];
   int maxlen = 0;
 #pragma unroll
@@ -292,7 +309,7 @@ __global__ __launch_bounds__(256) void clenshaw_lds_kernel(
 #pragma unroll
     for (int e = 0; e < V; ++e) acc[r][e] = T(0);
   if (full) {
-    constexpr int U = RT >= 4 ? 1 : (RT == 2 ? 2 : 4);
+    constexpr int U = PF ? (RT >= 4 ? 2 : 8 / RT) : (RT >= 4 ? 1 : (RT == 2 ? 2 : 4));
     for (int j = 0; j < maxlen; j += U) {
       int col[U][RT];
       T w[U][RT];
@@ -322,26 +339,22 @@ __global__ __launch_bounds__(256) void clenshaw_lds_kernel(
     for (int r = 0; r < RT; ++r) {
       const int row = row0 + r;
       if (row >= n) break;
+      const int rp = PF ? r : 0;
       T res[V];
 #pragma unroll
       for (int e = 0; e < V; ++e) res[e] = a1 * acc[r][e];
       if (Q != nullptr) {
-        T q[V];
-        ldv<T, V>(Q + (size_t)row * ldq, q);
+        if constexpr (!PF) ldv<T, V>(Q + (size_t)row * ldq, qv[0]);
 #pragma unroll
-        for (int e = 0; e < V; ++e) res[e] = (PERCOL ? pcq[c + e] : cq) * q[e] + res[e];
+        for (int e = 0; e < V; ++e) res[e] = (PERCOL ? pcq[c + e] : cq) * qv[rp][e] + res[e];
       }
-      {
-        T x[V];
-        ldv<T, V>(X1 + (size_t)row * ld1, x);
+      if constexpr (!PF) ldv<T, V>(X1 + (size_t)row * ld1, xv[0]);
 #pragma unroll
-        for (int e = 0; e < V; ++e) res[e] += b1 * x[e];
-      }
+      for (int e = 0; e < V; ++e) res[e] += b1 * xv[rp][e];
       if (X2 != nullptr) {
-        T y[V];
-        ldv_alias<T, V>(X2 + (size_t)row * ld2, y);
+        if constexpr (!PF) ldv_alias<T, V>(X2 + (size_t)row * ld2, yv[0]);
 #pragma unroll
-        for (int e = 0; e < V; ++e) res[e] -= s2 * y[e];
+        for (int e = 0; e < V; ++e) res[e] -= s2 * yv[rp][e];
       }
       stv<T, V>(out + (size_t)row * ldo, res);
     }
@@ -372,7 +385,7 @@ template <typename T, int V, int LPR, int RT, bool PERCOL>
 static void clenshaw_launch_t(hipStream_t st, const DevCSR& A, int b, const T* Q, int ldq,
                               const T* X1, int ld1, const T* P, int ldp, const T* X2, int ld2,
                               T* out, int ldo, T cq, T a1, T b1, T s2, const T* pcq, bool panels,
-                              bool lds) {
+                              int lds) {
   constexpr int RPB = 4 * (64 / LPR) * RT;
   const int nrowtiles = (A.n + RPB - 1) / RPB;
   const int nchunks = (b + LPR * V - 1) / (LPR * V);
@@ -385,9 +398,13 @@ static void clenshaw_launch_t(hipStream_t st, const DevCSR& A, int b, const T* Q
     grid = (int)nt;
   }
   const long long cs = panels ? (long long)A.n * (LPR * V) : 0LL;
-  if (lds && X1 != nullptr && P == nullptr)
-    hipLaunchKernelGGL((clenshaw_lds_kernel<T, V, LPR, RT, PERCOL>), dim3(grid), dim3(256), 0, st,
-                       A.n, b, nrowtiles, (int)nt, per_xcd, A.rowptr, A.col, A.vals<T>(), Q, ldq,
+  if (lds == 2 && RT <= 2 && X1 != nullptr && P == nullptr)
+    hipLaunchKernelGGL((clenshaw_lds_kernel<T, V, LPR, RT, PERCOL, (RT <= 2)>), dim3(grid), dim3(256),
+                       0, st, A.n, b, nrowtiles, (int)nt, per_xcd, A.rowptr, A.col, A.vals<T>(), Q,
+                       ldq, X1, ld1, X2, ld2, out, ldo, cq, a1, b1, s2, pcq, cs);
+  else if (lds && X1 != nullptr && P == nullptr)
+    hipLaunchKernelGGL((clenshaw_lds_kernel<T, V, LPR, RT, PERCOL, false>), dim3(grid), dim3(256), 0,
+                       st, A.n, b, nrowtiles, (int)nt, per_xcd, A.rowptr, A.col, A.vals<T>(), Q, ldq,
                        X1, ld1, X2, ld2, out, ldo, cq, a1, b1, s2, pcq, cs);
   else
     hipLaunchKernelGGL((clenshaw_kernel<T, V, LPR, RT, PERCOL>), dim3(grid), dim3(256), 0, st, A.n,
@@ -405,7 +422,7 @@ static void clenshaw_dispatch(hipStream_t st, const DevCSR& A, int b, const T* Q
   if (t.vec == VV && t.lpr == LL && t.rt == RR)                                                \
     return clenshaw_launch_t<T, VV, LL, RR, PERCOL>(st, A, b, Q, ldq, X1, ld1, P, ldp, X2, ld2, \
                                                     out, ldo, cq, a1, b1, s2, pcq, panels,  \
-                                                    t.lds != 0);
+                                                    t.lds);
 #define SMALGO_CL_V(VV, RR) \
   SMALGO_CL(VV, 4, RR) SMALGO_CL(VV, 8, RR) SMALGO_CL(VV, 16, RR) SMALGO_CL(VV, 32, RR) SMALGO_CL(VV, 64, RR)
   SMALGO_CL_V(1, 1) SMALGO_CL_V(2, 1) SMALGO_CL_V(4, 1) SMALGO_CL_V(8, 1)

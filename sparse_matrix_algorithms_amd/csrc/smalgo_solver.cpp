@@ -369,7 +369,8 @@ static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ld
       const int w = l * v;  // chunk width in elements
       if (w > 2 * std::max(b, 4) || w * (int)sizeof(T) > 2048) continue;
       for (int rt : {1, 2, 4})
-        for (int lds : {0, 1}) cand.push_back(FilterTile{v, l, rt, lds});
+        for (int lds : {0, 1, 2})
+          if (lds < 2 || rt <= 2) cand.push_back(FilterTile{v, l, rt, lds});
     }
   }
   if (cand.empty()) return def;
@@ -400,6 +401,16 @@ static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ld
       best_ms = tmin[i];
       best = cand[i];
     }
+  // SMALGO_TUNE_LOG=1: every candidate's time per step on stderr
+  const char* lg = std::getenv("SMALGO_TUNE_LOG");
+  if (lg && lg[0] == '1') {
+    for (size_t i = 0; i < cand.size(); ++i)
+      std::fprintf(stderr, "[smalgo tune] %s n=%d b=%d vec=%d lpr=%d rt=%d lds=%d: %.2f us\n",
+                   sizeof(T) == 4 ? "fp32" : "fp64", op.n, b, cand[i].vec, cand[i].lpr, cand[i].rt,
+                   cand[i].lds, tmin[i] * 1e3f / 6.0f);
+    std::fprintf(stderr, "[smalgo tune] picked vec=%d lpr=%d rt=%d lds=%d\n", best.vec, best.lpr,
+                 best.rt, best.lds);
+  }
   cx.tuned[key] = best;
   return best;
 }
@@ -1379,7 +1390,8 @@ struct SliceFilter {
         for (int l : {4, 8, 16, 32, 64})
           if (w % v == 0 && l * v <= std::max(w, 4) * 2 && l * v * (int)sizeof(T) <= 2048)
             for (int rt : {1, 2, 4})
-              for (int lds : {0, 1}) cand.push_back(FilterTile{v, l, rt, lds});
+              for (int lds : {0, 1, 2})
+                if (lds < 2 || rt <= 2) cand.push_back(FilterTile{v, l, rt, lds});
     }
     Event e0, e1;
     FilterTile best = cand.front();

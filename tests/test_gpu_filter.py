@@ -97,3 +97,40 @@ def test_filter_matvec_only(ctx):
     Y = sm.apply_poly_block(op, Q, co, 0.0, 4.0, ctx=ctx)
     Yo = P.apply_poly_block(A, Q, co, 0.0, 4.0)
     assert np.max(np.abs(Y - Yo)) <= 1e-13 * np.max(np.abs(Yo))
+
+
+def _tiles():
+    for v in (1, 2, 4, 8):
+        for l in (4, 8, 16, 32, 64):
+            for rt in (1, 2, 4):
+                for lds in (0, 1, 2):
+                    if lds < 2 or rt <= 2:
+                        yield v, l, rt, lds
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_filter_every_tile(ctx, prec, monkeypatch):
+    """Every filter tile the autotuner may pick (lane vector, lanes per row, rows
+    per team; CSR per lane, staged in LDS, or staged with own-row prefetch),
+    forced one at a time, against the FP64 oracle: ragged XXZ rows (1..6
+    nonzeros), a random symmetric matrix with ~24 nonzeros per row (more than
+    the LDS tile holds, so staging falls back to global loads), a full and a
+    ragged column chunk."""
+    import scipy.sparse as sps
+
+    import sparse_matrix_algorithms_amd as sm
+    R = sps.random(300, 300, density=0.04, random_state=np.random.default_rng(5), format="csr")
+    mats = [xxz_sz0(10, 5.0, 3).tocsr(), (R + R.T + sps.identity(300)).tocsr()]
+    tol = 1e-12 if prec == "fp64" else 2e-4
+    for A in mats:
+        smin, smax = P.gershgorin_bounds(A)
+        dm = sm.DeviceMatrix(A, ctx)
+        co = P.build_coeffs(20, smin, smax, 0.5 * (smin + smax), smin * 0.3, smax * 0.3, "step")
+        for b in (64, 40):
+            Q = gaussian_block(6, A.shape[0], b)
+            Yo = P.apply_poly_block(A, Q, co, smin, smax)
+            for v, l, rt, lds in _tiles():
+                monkeypatch.setenv("SMALGO_FILTER_TILE", f"{v},{l},{rt},{lds}")
+                Y = sm.apply_poly_block(dm, Q, co, smin, smax, precision=prec, ctx=ctx)
+                err = np.linalg.norm(Y - Yo) / np.linalg.norm(Yo)
+                assert err <= tol, (A.shape, b, v, l, rt, lds, err)
