@@ -375,19 +375,27 @@ static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ld
   }
   if (cand.empty()) return def;
   Event e0, e1;
-  // Two sweeps over the candidates, each timing 6 steps of the general form
-  // out = cq Q + a1 A X1 + b1 X1 - X2 after one warm step; a tile's time is
-  // its faster sweep, so clock ramps and one-off stalls do not pick the tile.
+  // The steps timed are the ones the recurrence runs from its third step on:
+  // Q, X1 = buf0 and X2 = buf1 distinct, the output written in place over X2
+  // (out = cq Q + a1 A X1 + b1 X1 - X2).  The two recurrence steps without X2
+  // first give buf0 and buf1 defined values; the in-place step then maps buf1
+  // to c - buf1, so the values stay bounded.  Two sweeps over the candidates,
+  // each timing 6 steps after one warm step; a tile's time is its faster
+  // sweep, so clock ramps and one-off stalls do not pick the tile.
+  launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), Q, ldq, T(0.1), T(0.1), nullptr, ldb,
+                          T(0), buf0, ldb, def);
+  launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), buf0, ldb, T(0.1), T(0.1), nullptr,
+                          ldb, T(0), buf1, ldb, def);
   std::vector<float> tmin(cand.size(), 1e30f);
   for (int sweep = 0; sweep < 2; ++sweep)
     for (size_t i = 0; i < cand.size(); ++i) {
       const FilterTile& t = cand[i];
-      launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), Q, ldq, T(0.1), T(0.1), buf1,
-                              ldb, T(1), buf0, ldb, t);
+      launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), buf0, ldb, T(0.1), T(0.1), buf1,
+                              ldb, T(1), buf1, ldb, t);
       SMALGO_HIP(hipEventRecord(e0, cx.stream));
       for (int r = 0; r < 6; ++r)
-        launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), Q, ldq, T(0.1), T(0.1), buf1,
-                                ldb, T(1), buf0, ldb, t);
+        launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), buf0, ldb, T(0.1), T(0.1),
+                                buf1, ldb, T(1), buf1, ldb, t);
       SMALGO_HIP(hipEventRecord(e1, cx.stream));
       SMALGO_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
