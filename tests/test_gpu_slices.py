@@ -45,6 +45,26 @@ def test_slices_filter_fp64_vs_oracle(ctx, pw):
             assert err <= 1e-12, (r, i, err)
 
 
+@pytest.mark.parametrize("tile", ["4,4,1,1", "2,4,2,2", "1,8,4,0", "4,4,1,2", "8,4,1,1"])
+def test_slices_filter_forced_tiles_vs_oracle(ctx, tile, monkeypatch):
+    """Packed panels (dense n x 32 slabs) with forced tiles: column chunks of
+    8 to 32 columns, CSR per lane, staged in LDS, and staged with own-row
+    prefetch."""
+    import sparse_matrix_algorithms_amd as sm
+    monkeypatch.setenv("SMALGO_SLICE_TILE", tile)
+    A = anderson3d(8, 16.5, 5)
+    n = A.shape[0]
+    smin, smax = P.gershgorin_bounds(A)
+    co = _slice_coeffs(smin, smax)
+    blocks = [gaussian_block(300 + i, n, w) for i, w in enumerate(WIDTHS)]
+    outs = sm.apply_poly_slices(A, [blocks], co, smin, smax, precision="fp64", panel_width=32,
+                                use_graph=True, ctx=ctx)
+    for i in range(len(WIDTHS)):
+        ref = P.apply_poly_block(A, blocks[i], co[i], smin, smax)
+        err = np.abs(outs[0][i] - ref).max() / max(np.abs(ref).max(), 1e-300)
+        assert err <= 1e-12, (tile, i, err)
+
+
 def test_slices_filter_graph_equals_eager(ctx):
     import sparse_matrix_algorithms_amd as sm
     A = anderson3d(8, 16.5, 4)
