@@ -6,7 +6,6 @@ import json
 from pathlib import Path
 
 import numpy as np
-import pytest
 
 from oracle.polfed_ref import gershgorin_bounds
 from sparse_matrix_algorithms_amd.lattices import xxz_sz0
@@ -14,8 +13,6 @@ from sparse_matrix_algorithms_amd.lattices import xxz_sz0
 GOLDEN = Path(__file__).resolve().parent / "golden" / "xxz_L20_h5_seed20_centre.json"
 
 
-@pytest.mark.skipif(not GOLDEN.exists(), reason="fixture not generated yet "
-                    "(golden/make_xxz20_golden.py)")
 def test_xxz20_fixture():
     g = json.loads(GOLDEN.read_text())
     A = xxz_sz0(g["L"], g["h"], g["seed"])

@@ -18,8 +18,7 @@ import numpy as np
 import pytest
 
 GOLDEN = Path(__file__).resolve().parent / "golden" / "xxz_L20_h5_seed20_centre.json"
-pytestmark = [pytest.mark.gpu, pytest.mark.skipif(
-    not GOLDEN.exists(), reason="fixture not generated yet (golden/make_xxz20_golden.py)")]
+pytestmark = pytest.mark.gpu
 
 
 def test_xxz20_polfed_mixed_against_oracle_fixture(ctx):
