@@ -116,7 +116,7 @@ def test_anderson48_polfed_mixed_and_fp64_full_size(ctx, anderson48):
         model = MLP.load(ctx=ctx)
         common = dict(s=64, smin=smin, smax=smax, sigma=0.0, emin=emin, emax=emax,
                       tol=1e-10 * anorm, ctx=ctx, seed=1)
-        sets = {}
+        sets, maxres = {}, {}
         for prec in ("fp64", "fp32"):
             K, mm, _ = choose_parameters(model, mu[:64], n, 500, emin, emax, smin, smax, 64,
                                          precision=prec)
@@ -132,6 +132,7 @@ def test_anderson48_polfed_mixed_and_fp64_full_size(ctx, anderson48):
             G = X.T @ X
             assert np.max(np.abs(G - np.eye(len(lam)))) <= 1e-8, prec
             sets[prec] = np.sort(lam)
+            maxres[prec] = float(res.max() / anorm)
         a, b = sets["fp64"], sets["fp32"]
         assert len(a) == len(b)
         assert np.max(np.abs(a - b)) <= 1e-10 * anorm
@@ -143,5 +144,9 @@ def test_anderson48_polfed_mixed_and_fp64_full_size(ctx, anderson48):
         exact = ev[(ev >= emin) & (ev <= emax)]
         assert len(a) == len(exact), (len(a), len(exact))
         assert np.max(np.abs(a - exact)) <= 1e-10 * anorm
+        for prec in ("fp64", "fp32"):
+            print(f"C2 {prec}: {len(sets[prec])} pairs, max |dl| "
+                  f"{np.max(np.abs(sets[prec] - exact)):.1e}, max rel res {maxres[prec]:.1e}",
+                  flush=True)
     finally:
         dm.free()

@@ -36,6 +36,8 @@ def test_anderson20_chebyshev_fp64_matches_dense_eigh(ctx):
     assert len(lam) == len(ref), (len(lam), len(ref))
     assert np.max(np.abs(lam - ref)) <= 1e-10 * anorm
     assert np.max(res) <= 1e-10 * anorm
+    print(f"C1 chebyshev: {len(lam)} pairs, max |dl| {np.max(np.abs(lam - ref)):.1e}, "
+          f"max rel res {np.max(res) / anorm:.1e}", flush=True)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
