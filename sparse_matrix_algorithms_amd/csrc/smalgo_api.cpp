@@ -245,29 +245,44 @@ int smalgo_gershgorin_bounds(const smalgo_mat* m, double* smin, double* smax) {
 }
 
 // ---- coefficients --------------------------------------------------------
-static int put(const std::vector<double>& v, double* out) {
+// The coefficient calls have no context to hold a message: an exception
+// (allocation failure for a huge K or nsamp) becomes a status code here and
+// never crosses the C ABI.
+template <typename F>
+static int put(double* out, F&& make) {
   if (!out) return SMALGO_ERR_ARG;
-  std::copy(v.begin(), v.end(), out);
-  return SMALGO_OK;
+  try {
+    const std::vector<double> v = make();
+    std::copy(v.begin(), v.end(), out);
+    return SMALGO_OK;
+  } catch (const Error& e) {
+    return e.code;
+  } catch (...) {
+    return SMALGO_ERR_INTERNAL;
+  }
 }
-int smalgo_jackson_kernel(int K, double* g) { return put(jackson_kernel(K), g); }
+int smalgo_jackson_kernel(int K, double* g) {
+  return put(g, [&] { return jackson_kernel(K); });
+}
 int smalgo_cheb_delta_coeffs(double sh, int K, int jackson, double* c) {
   if (K < 0) return SMALGO_ERR_ARG;
-  return put(cheb_delta_coeffs(sh, K, jackson != 0), c);
+  return put(c, [&] { return cheb_delta_coeffs(sh, K, jackson != 0); });
 }
 int smalgo_cheb_window_coeffs(double a, double b, int K, int jackson, double* c) {
   if (K < 0) return SMALGO_ERR_ARG;
-  return put(cheb_window_coeffs(a, b, K, jackson != 0), c);
+  return put(c, [&] { return cheb_window_coeffs(a, b, K, jackson != 0); });
 }
 int smalgo_cheb_smooth_window_coeffs(double a, double b, int K, int jackson, int nsamp,
                                      double delta, double* c) {
   if (K < 0) return SMALGO_ERR_ARG;
-  return put(cheb_smooth_window_coeffs(a, b, K, jackson != 0, nsamp, delta), c);
+  return put(c, [&] { return cheb_smooth_window_coeffs(a, b, K, jackson != 0, nsamp, delta); });
 }
 int smalgo_build_coeffs(int degree, double smin, double smax, double sigma, double emin,
                         double emax, int window_type, int jackson, double* c) {
   if (degree < 0 || window_type < 0 || window_type > 2) return SMALGO_ERR_ARG;
-  return put(build_coeffs(degree, smin, smax, sigma, emin, emax, window_type, jackson != 0), c);
+  return put(c, [&] {
+    return build_coeffs(degree, smin, smax, sigma, emin, emax, window_type, jackson != 0);
+  });
 }
 
 // ---- filter ---------------------------------------------------------------
