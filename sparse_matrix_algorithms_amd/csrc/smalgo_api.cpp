@@ -248,6 +248,7 @@ int smalgo_gershgorin_bounds(const smalgo_mat* m, double* smin, double* smax) {
 // The coefficient calls have no context to hold a message: an exception
 // (allocation failure for a huge K or nsamp) becomes a status code here and
 // never crosses the C ABI.
+extern "C++" {
 template <typename F>
 static int put(double* out, F&& make) {
   if (!out) return SMALGO_ERR_ARG;
@@ -260,6 +261,7 @@ static int put(double* out, F&& make) {
   } catch (...) {
     return SMALGO_ERR_INTERNAL;
   }
+}
 }
 int smalgo_jackson_kernel(int K, double* g) {
   return put(g, [&] { return jackson_kernel(K); });
