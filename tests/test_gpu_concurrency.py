@@ -1,0 +1,73 @@
+"""Two contexts on one GPU driven from two host threads at once (ctypes drops
+the GIL inside every library call): each thread's POLFED solves (FP64 and
+mixed) and its large Rayleigh–Ritz result downloads (>= 128 MiB, the pinned
+chunked path whose host buffers the threads share under a lock) give the same
+answers as the same calls made one after the other."""
+import math
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from oracle import polfed_ref as P
+from sparse_matrix_algorithms_amd.lattices import anderson3d, gaussian_block
+
+pytestmark = pytest.mark.gpu
+
+
+def _job(sm, ctx, A, Abig, Y):
+    """Every call of one thread: returns the results to compare."""
+    smin, smax = P.gershgorin_bounds(A)
+    anorm = max(abs(smin), abs(smax))
+    ev = np.linalg.eigh(A.toarray())[0]
+    i0 = np.searchsorted(ev, 0.0)
+    emin, emax = 0.5 * (ev[i0 - 16] + ev[i0 - 15]), 0.5 * (ev[i0 + 14] + ev[i0 + 15])
+    h = 0.5 * (emax - emin) / (0.5 * (smax - smin))
+    out = {}
+    dm = sm.DeviceMatrix(A, ctx)
+    try:
+        for prec in ("fp64", "fp32"):
+            lam, X, res = sm.polfed(dm, 36, math.ceil(8 / h), s=16, smin=smin, smax=smax,
+                                    emin=emin, emax=emax, mmax=9, tol=1e-10 * anorm,
+                                    precision=prec, ctx=ctx)
+            out[prec] = (np.sort(lam), float(res.max() / anorm))
+    finally:
+        dm.free()
+    out["exact"] = ev[(ev >= emin) & (ev <= emax)]
+    lam, X, res = sm.subspace_rr(Abig, Y, ctx=ctx)   # n x 160 FP64 = 142 MB download
+    out["rr"] = (lam, X, res)
+    return out
+
+
+def test_two_contexts_two_threads(ctx):
+    import sparse_matrix_algorithms_amd as sm
+    ctxs = [sm.Context(0), sm.Context(0)]
+    mats = [anderson3d(12, 16.5, 31), anderson3d(12, 16.5, 32)]
+    big = anderson3d(48, 16.5, 48)
+    Ys = [gaussian_block(41, big.shape[0], 160), gaussian_block(42, big.shape[0], 160)]
+    try:
+        seq = [_job(sm, c, A, big, Y) for c, A, Y in zip(ctxs, mats, Ys)]
+        with ThreadPoolExecutor(2) as ex:
+            futs = [ex.submit(_job, sm, c, A, big, Y) for c, A, Y in zip(ctxs, mats, Ys)]
+            par = [f.result(timeout=300) for f in futs]
+    finally:
+        for c in ctxs:
+            c.close()
+    for i, A in enumerate(mats):
+        smin, smax = P.gershgorin_bounds(A)
+        anorm = max(abs(smin), abs(smax))
+        exact = seq[i]["exact"]
+        assert len(exact) == 30
+        for prec in ("fp64", "fp32"):
+            for run in (seq[i], par[i]):
+                lam, maxres = run[prec]
+                assert len(lam) == len(exact), (i, prec, len(lam))
+                assert np.max(np.abs(lam - exact)) <= 1e-10 * anorm, (i, prec)
+                assert maxres <= 1e-10, (i, prec, maxres)
+        ls, Xs, rs = seq[i]["rr"]
+        lp, Xp, rp = par[i]["rr"]
+        assert np.max(np.abs(ls - lp)) <= 1e-12 * np.max(np.abs(ls))
+        # the same Ritz vectors up to sign
+        sg = np.sign(np.sum(Xs * Xp, axis=0))
+        assert np.max(np.abs(Xs - Xp * sg[None, :])) <= 1e-8
+        np.testing.assert_allclose(rp, rs, rtol=1e-6, atol=1e-12)
