@@ -1401,25 +1401,43 @@ struct SliceFilter {
               for (int lds : {0, 1, 2})
                 if (lds < 2 || rt <= 2) cand.push_back(FilterTile{v, l, rt, lds});
     }
+    // The step the recurrence runs: Q, X1 and X2 distinct, the output in
+    // place over X2 (out = cq∘Q + a1 A X1 + b1 X1 - X2).  Two sweeps of 4
+    // steps after a warm step; a tile's time is its faster sweep.
+    DBuf<T> qt((size_t)n * w);
+    fill_block<T>(cx.stream, n, w, qt.p, w, T(0));
     Event e0, e1;
+    std::vector<float> tmin(cand.size(), 1e30f);
+    for (int sweep = 0; sweep < (cand.size() > 1 ? 2 : 1); ++sweep)
+      for (size_t i = 0; i < cand.size(); ++i) {
+        const FilterTile& tl = cand[i];
+        launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, qt.p, w, co.p, P0.p, w, T(0.1), T(0.1),
+                                     P1.p, w, T(1), P1.p, w, tl);
+        SMALGO_HIP(hipEventRecord(e0, cx.stream));
+        for (int r = 0; r < 4; ++r)
+          launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, qt.p, w, co.p, P0.p, w, T(0.1),
+                                       T(0.1), P1.p, w, T(1), P1.p, w, tl);
+        SMALGO_HIP(hipEventRecord(e1, cx.stream));
+        SMALGO_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
+        tmin[i] = std::min(tmin[i], ms / 4);
+      }
     FilterTile best = cand.front();
     float best_ms = 1e30f;
-    for (const FilterTile& tl : cand) {
-      // the general step out = cq∘Q + a1 A X1 + b1 X1 - X2, in place on X2
-      launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, P1.p, w, co.p, P0.p, w, T(0.1), T(0.1),
-                                   P1.p, w, T(1), P1.p, w, tl);
-      SMALGO_HIP(hipEventRecord(e0, cx.stream));
-      for (int r = 0; r < 4; ++r)
-        launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, P1.p, w, co.p, P0.p, w, T(0.1), T(0.1),
-                                     P1.p, w, T(1), P1.p, w, tl);
-      SMALGO_HIP(hipEventRecord(e1, cx.stream));
-      SMALGO_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms / 4 < best_ms) {
-        best_ms = ms / 4;
-        best = tl;
+    for (size_t i = 0; i < cand.size(); ++i)
+      if (tmin[i] < best_ms) {
+        best_ms = tmin[i];
+        best = cand[i];
       }
+    const char* lg = std::getenv("SMALGO_TUNE_LOG");
+    if (lg && lg[0] == '1') {
+      for (size_t i = 0; i < cand.size(); ++i)
+        std::fprintf(stderr, "[smalgo tune] panel %s n=%d w=%d vec=%d lpr=%d rt=%d lds=%d: %.2f us\n",
+                     sizeof(T) == 4 ? "fp32" : "fp64", n, w, cand[i].vec, cand[i].lpr, cand[i].rt,
+                     cand[i].lds, tmin[i] * 1e3f);
+      std::fprintf(stderr, "[smalgo tune] panel picked vec=%d lpr=%d rt=%d lds=%d\n", best.vec,
+                   best.lpr, best.rt, best.lds);
     }
     cx.tuned[key] = best;
     cx.tuned_ms[key] = best_ms;
