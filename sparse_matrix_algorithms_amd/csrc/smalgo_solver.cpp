@@ -860,6 +860,16 @@ std::string format_julia_float(double v) {
   return out;
 }
 
+// Julia's @printf("%.3e", x): C's %.3e, except that infinities and NaN are
+// written Inf, -Inf and NaN.
+static std::string printf_e3(double x) {
+  if (std::isnan(x)) return "NaN";
+  if (std::isinf(x)) return x > 0 ? "Inf" : "-Inf";
+  char buf[32];
+  std::snprintf(buf, sizeof buf, "%.3e", x);
+  return buf;
+}
+
 std::string progress_line(int m, const std::vector<double>& lam, const std::vector<double>& res) {
   if (res.empty()) return "m=" + std::to_string(m) + "  inside=0  maxres=Inf";
   std::vector<double> rs(res);
@@ -2401,13 +2411,37 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
       }
     if (cnt == 0) maxres = std::numeric_limits<double>::infinity();
     if (p.debug) {
-      std::printf("it=%d inside=%d maxres=%.3e\n", it, cnt, maxres);
+      if (matrix_free) {
+        // the matrix-free form adds quantile(res[inside], 0.9) (:826): Julia's
+        // default quantile, linear between order statistics at h = (k-1)·0.9
+        std::vector<double> ri;
+        for (int j = 0; j < m0; ++j)
+          if (lam[j] >= emin && lam[j] <= emax && res[j] < p.res_gate) ri.push_back(res[j]);
+        double r90 = std::numeric_limits<double>::infinity();
+        if (!ri.empty()) {
+          std::sort(ri.begin(), ri.end());
+          const double h = (double)(ri.size() - 1) * 0.9;
+          const size_t lo = (size_t)std::floor(h);
+          const size_t hi = std::min(lo + 1, ri.size() - 1);
+          r90 = ri[lo] + (h - (double)lo) * (ri[hi] - ri[lo]);
+        }
+        std::printf("it=%d inside=%d maxres=%s r90=%s\n", it, cnt, printf_e3(maxres).c_str(),
+                    printf_e3(r90).c_str());
+      } else {
+        std::printf("it=%d inside=%d maxres=%s\n", it, cnt, printf_e3(maxres).c_str());
+      }
       std::fflush(stdout);
     }
     if (matrix_free) {  // :827-836
       const bool ok = cnt > 0 && maxres < p.tol;
       if (p.two_hit) {
-        if (ok && (it == 1 || prev_ok)) break;
+        if (ok && (it == 1 || prev_ok)) {
+          if (p.debug) {
+            std::printf("converged\n");
+            std::fflush(stdout);
+          }
+          break;
+        }
         prev_ok = ok;
       } else if (ok) {
         break;
