@@ -353,6 +353,61 @@ static long long tile_key(const Op& op, int b, const T* Q, int ldq, const T* buf
          (al(8) << 3) ^ (al(4) << 2) ^ (al(2) << 1);
 }
 
+// Every tile the autotuners consider at block width b: lane vectors of 1, 2,
+// 4 or 8 elements that vec_ok admits (operand alignment, width), column
+// chunks lpr·vec of at most twice the width and at most 2 KB, 1, 2 or 4 rows
+// per team, and the three CSR forms (per lane, staged in LDS, staged with
+// own-row prefetch; the last for 1 or 2 rows per team).
+template <typename VecOk>
+static std::vector<FilterTile> tile_candidates(int b, size_t elem, VecOk&& vec_ok) {
+  std::vector<FilterTile> cand;
+  for (int v : {1, 2, 4, 8}) {
+    if (!vec_ok(v)) continue;
+    for (int l : {4, 8, 16, 32, 64}) {
+      const int w = l * v;  // chunk width in elements
+      if (w > 2 * std::max(b, 4) || w * (int)elem > 2048) continue;
+      for (int rt : {1, 2, 4})
+        for (int lds : {0, 1, 2})
+          if (lds < 2 || rt <= 2) cand.push_back(FilterTile{v, l, rt, lds});
+    }
+  }
+  return cand;
+}
+
+// Times step(tile) for every candidate in `sweeps` sweeps of `reps` steps, each
+// after one warm step; a tile's time (ms per step, into tmin) is its fastest
+// sweep, so clock ramps and one-off stalls do not pick the tile.  Returns the
+// index of the fastest.  SMALGO_TUNE_LOG=1 prints every time and the pick.
+template <typename StepFn>
+static size_t time_tiles(Ctx& cx, const std::vector<FilterTile>& cand, int sweeps, int reps,
+                         StepFn&& step, std::vector<float>& tmin, const std::string& what) {
+  Event e0, e1;
+  tmin.assign(cand.size(), 1e30f);
+  for (int sweep = 0; sweep < sweeps; ++sweep)
+    for (size_t i = 0; i < cand.size(); ++i) {
+      step(cand[i]);
+      SMALGO_HIP(hipEventRecord(e0, cx.stream));
+      for (int r = 0; r < reps; ++r) step(cand[i]);
+      SMALGO_HIP(hipEventRecord(e1, cx.stream));
+      SMALGO_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
+      tmin[i] = std::min(tmin[i], ms / (float)reps);
+    }
+  size_t best = 0;
+  for (size_t i = 1; i < cand.size(); ++i)
+    if (tmin[i] < tmin[best]) best = i;
+  const char* lg = std::getenv("SMALGO_TUNE_LOG");
+  if (lg && lg[0] == '1') {
+    for (size_t i = 0; i < cand.size(); ++i)
+      std::fprintf(stderr, "[smalgo tune] %s vec=%d lpr=%d rt=%d lds=%d: %.2f us\n", what.c_str(),
+                   cand[i].vec, cand[i].lpr, cand[i].rt, cand[i].lds, tmin[i] * 1e3f);
+    std::fprintf(stderr, "[smalgo tune] %s picked vec=%d lpr=%d rt=%d lds=%d\n", what.c_str(),
+                 cand[best].vec, cand[best].lpr, cand[best].rt, cand[best].lds);
+  }
+  return best;
+}
+
 template <typename T>
 static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ldq, T* buf0, T* buf1,
                                 int ldb, T* Y, int ldy) {
@@ -362,63 +417,29 @@ static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ld
   const long long key = tile_key<T>(op, b, Q, ldq, buf0, buf1, ldb, Y, ldy);
   auto it = cx.tuned.find(key);
   if (it != cx.tuned.end()) return it->second;
-  std::vector<FilterTile> cand;
-  for (int v : {1, 2, 4, 8}) {
-    if (!al(v) || (v > 1 && b < v)) continue;
-    for (int l : {4, 8, 16, 32, 64}) {
-      const int w = l * v;  // chunk width in elements
-      if (w > 2 * std::max(b, 4) || w * (int)sizeof(T) > 2048) continue;
-      for (int rt : {1, 2, 4})
-        for (int lds : {0, 1, 2})
-          if (lds < 2 || rt <= 2) cand.push_back(FilterTile{v, l, rt, lds});
-    }
-  }
+  const std::vector<FilterTile> cand =
+      tile_candidates(b, sizeof(T), [&](int v) { return al(v) && (v == 1 || b >= v); });
   if (cand.empty()) return def;
-  Event e0, e1;
   // The steps timed are the ones the recurrence runs from its third step on:
   // Q, X1 = buf0 and X2 = buf1 distinct, the output written in place over X2
   // (out = cq Q + a1 A X1 + b1 X1 - X2).  The two recurrence steps without X2
   // first give buf0 and buf1 defined values; the in-place step then maps buf1
-  // to c - buf1, so the values stay bounded.  Two sweeps over the candidates,
-  // each timing 6 steps after one warm step; a tile's time is its faster
-  // sweep, so clock ramps and one-off stalls do not pick the tile.
+  // to c - buf1, so the values stay bounded.  Two sweeps of 6 steps.
   launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), Q, ldq, T(0.1), T(0.1), nullptr, ldb,
                           T(0), buf0, ldb, def);
   launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), buf0, ldb, T(0.1), T(0.1), nullptr,
                           ldb, T(0), buf1, ldb, def);
-  std::vector<float> tmin(cand.size(), 1e30f);
-  for (int sweep = 0; sweep < 2; ++sweep)
-    for (size_t i = 0; i < cand.size(); ++i) {
-      const FilterTile& t = cand[i];
-      launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), buf0, ldb, T(0.1), T(0.1), buf1,
-                              ldb, T(1), buf1, ldb, t);
-      SMALGO_HIP(hipEventRecord(e0, cx.stream));
-      for (int r = 0; r < 6; ++r)
+  std::vector<float> tmin;
+  const size_t ib = time_tiles(
+      cx, cand, 2, 6,
+      [&](const FilterTile& t) {
         launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), buf0, ldb, T(0.1), T(0.1),
                                 buf1, ldb, T(1), buf1, ldb, t);
-      SMALGO_HIP(hipEventRecord(e1, cx.stream));
-      SMALGO_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
-      tmin[i] = std::min(tmin[i], ms);
-    }
-  FilterTile best = def;
-  float best_ms = 1e30f;
-  for (size_t i = 0; i < cand.size(); ++i)
-    if (tmin[i] < best_ms) {
-      best_ms = tmin[i];
-      best = cand[i];
-    }
-  // SMALGO_TUNE_LOG=1: every candidate's time per step on stderr
-  const char* lg = std::getenv("SMALGO_TUNE_LOG");
-  if (lg && lg[0] == '1') {
-    for (size_t i = 0; i < cand.size(); ++i)
-      std::fprintf(stderr, "[smalgo tune] %s n=%d b=%d vec=%d lpr=%d rt=%d lds=%d: %.2f us\n",
-                   sizeof(T) == 4 ? "fp32" : "fp64", op.n, b, cand[i].vec, cand[i].lpr, cand[i].rt,
-                   cand[i].lds, tmin[i] * 1e3f / 6.0f);
-    std::fprintf(stderr, "[smalgo tune] picked vec=%d lpr=%d rt=%d lds=%d\n", best.vec, best.lpr,
-                 best.rt, best.lds);
-  }
+      },
+      tmin,
+      std::string(sizeof(T) == 4 ? "fp32" : "fp64") + " n=" + std::to_string(op.n) +
+          " b=" + std::to_string(b));
+  const FilterTile best = cand[ib];
   cx.tuned[key] = best;
   return best;
 }
@@ -1404,51 +1425,24 @@ struct SliceFilter {
       const FilterTile d = choose_tile(sizeof(T), w, {P0.p, P1.p}, {w});
       cand.push_back(FilterTile{d.vec, d.lpr, 1});
     } else {
-      for (int v : {1, 2, 4, 8})
-        for (int l : {4, 8, 16, 32, 64})
-          if (w % v == 0 && l * v <= std::max(w, 4) * 2 && l * v * (int)sizeof(T) <= 2048)
-            for (int rt : {1, 2, 4})
-              for (int lds : {0, 1, 2})
-                if (lds < 2 || rt <= 2) cand.push_back(FilterTile{v, l, rt, lds});
+      cand = tile_candidates(w, sizeof(T), [&](int v) { return w % v == 0; });
     }
     // The step the recurrence runs: Q, X1 and X2 distinct, the output in
-    // place over X2 (out = cq∘Q + a1 A X1 + b1 X1 - X2).  Two sweeps of 4
-    // steps after a warm step; a tile's time is its faster sweep.
+    // place over X2 (out = cq∘Q + a1 A X1 + b1 X1 - X2); two sweeps of 4 steps.
     DBuf<T> qt((size_t)n * w);
     fill_block<T>(cx.stream, n, w, qt.p, w, T(0));
-    Event e0, e1;
-    std::vector<float> tmin(cand.size(), 1e30f);
-    for (int sweep = 0; sweep < (cand.size() > 1 ? 2 : 1); ++sweep)
-      for (size_t i = 0; i < cand.size(); ++i) {
-        const FilterTile& tl = cand[i];
-        launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, qt.p, w, co.p, P0.p, w, T(0.1), T(0.1),
-                                     P1.p, w, T(1), P1.p, w, tl);
-        SMALGO_HIP(hipEventRecord(e0, cx.stream));
-        for (int r = 0; r < 4; ++r)
-          launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, qt.p, w, co.p, P0.p, w, T(0.1),
-                                       T(0.1), P1.p, w, T(1), P1.p, w, tl);
-        SMALGO_HIP(hipEventRecord(e1, cx.stream));
-        SMALGO_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
-        tmin[i] = std::min(tmin[i], ms / 4);
-      }
-    FilterTile best = cand.front();
-    float best_ms = 1e30f;
-    for (size_t i = 0; i < cand.size(); ++i)
-      if (tmin[i] < best_ms) {
-        best_ms = tmin[i];
-        best = cand[i];
-      }
-    const char* lg = std::getenv("SMALGO_TUNE_LOG");
-    if (lg && lg[0] == '1') {
-      for (size_t i = 0; i < cand.size(); ++i)
-        std::fprintf(stderr, "[smalgo tune] panel %s n=%d w=%d vec=%d lpr=%d rt=%d lds=%d: %.2f us\n",
-                     sizeof(T) == 4 ? "fp32" : "fp64", n, w, cand[i].vec, cand[i].lpr, cand[i].rt,
-                     cand[i].lds, tmin[i] * 1e3f);
-      std::fprintf(stderr, "[smalgo tune] panel picked vec=%d lpr=%d rt=%d lds=%d\n", best.vec,
-                   best.lpr, best.rt, best.lds);
-    }
+    std::vector<float> tmin;
+    const size_t ib = time_tiles(
+        cx, cand, cand.size() > 1 ? 2 : 1, 4,
+        [&](const FilterTile& tl) {
+          launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, qt.p, w, co.p, P0.p, w, T(0.1), T(0.1),
+                                       P1.p, w, T(1), P1.p, w, tl);
+        },
+        tmin,
+        std::string("panel ") + (sizeof(T) == 4 ? "fp32" : "fp64") + " n=" + std::to_string(n) +
+            " w=" + std::to_string(w));
+    const FilterTile best = cand[ib];
+    const float best_ms = tmin[ib];
     cx.tuned[key] = best;
     cx.tuned_ms[key] = best_ms;
     *ms_out = best_ms;
