@@ -193,3 +193,51 @@ def test_chebyshev_rejects_nonsymmetric_matrix(ctx):
     A = sp.csr_matrix(np.array([[2.0, 1.0, 0.0], [0.0, 2.0, 1.0], [0.0, 0.0, 2.0]]))
     with pytest.raises(ValueError, match="symmetric"):
         sm.chebyshev_poly_real_symm_sparse(A, 1.0, 3.0, m0=2, degree=4, ctx=ctx)
+
+
+@pytest.mark.parametrize("s", [1, 3, 5, 12])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_polfed_block_sizes_off_the_vector_width(ctx, s, precision):
+    """Block sizes that are not a multiple of the 4-element row padding (the
+    reference takes any s >= 1, polfed.jl:1128): the same eigenvalues as dense
+    eigh and as the oracle from the same start block."""
+    import sparse_matrix_algorithms_amd as sm
+    A = anderson3d(8, 16.5, 40)
+    n = A.shape[0]
+    ev = np.linalg.eigh(A.toarray())[0]
+    i0 = np.searchsorted(ev, 0.0)
+    emin, emax = 0.5 * (ev[i0 - 7] + ev[i0 - 6]), 0.5 * (ev[i0 + 5] + ev[i0 + 6])
+    exact = ev[(ev >= emin) & (ev <= emax)]
+    smin, smax = P.gershgorin_bounds(A)
+    anorm = max(abs(smin), abs(smax))
+    nev = len(exact)
+    K = math.ceil(8 / (0.5 * (emax - emin) / (0.5 * (smax - smin))))
+    kw = dict(s=s, smin=smin, smax=smax, emin=emin, emax=emax, mmax=math.ceil(3 * nev / s) + 4,
+              tol=1e-10 * anorm)
+    Q1 = gaussian_block(2, n, s)
+    lam, X, res = sm.polfed(A, nev, K, Q1=Q1, precision=precision, ctx=ctx, **kw)
+    lo, _, ro = P.polfed(A, nev, K, Q0=Q1, **kw)
+    assert len(lam) == len(lo) == nev == 12, (len(lam), len(lo))
+    assert np.max(np.abs(np.sort(lam) - exact)) <= 1e-10 * anorm
+    assert np.max(np.abs(np.sort(lam) - np.sort(lo))) <= 1e-10 * anorm
+    assert res.max() <= 1e-10 * anorm
+    assert np.max(np.abs(X.T @ X - np.eye(nev))) <= 1e-8
+
+
+def test_chebyshev_m0_beyond_n(ctx):
+    """m0 > n: the reference keeps min(n, m0) columns (thin QR of an n x m0
+    block, chebyshev_subspace_iteration.jl:733-734), so the subspace is the
+    whole space and the window's eigenpairs come out exactly."""
+    import sparse_matrix_algorithms_amd as sm
+    from oracle import cheb_ref as CR
+    n = 40
+    A = lap1d(n)
+    lex = lap1d_exact_eigs(n)
+    emin, emax = 0.5 * (lex[15] + lex[16]), 0.5 * (lex[21] + lex[22])
+    kw = dict(smin=0.0, smax=4.0, m0=60, degree=20, maxiter=2)
+    lam, X, res = sm.chebyshev_poly_real_symm_sparse(A, emin, emax, ctx=ctx, **kw)
+    lo, _, ro = CR.chebyshev_poly_real_symm_sparse(A, emin, emax, **kw)
+    ref = lex[(lex >= emin) & (lex <= emax)]
+    assert len(lam) == len(lo) == len(ref) == 6
+    assert np.max(np.abs(lam - ref)) <= 1e-12 and np.max(np.abs(lo - ref)) <= 1e-12
+    assert res.max() <= 1e-10
