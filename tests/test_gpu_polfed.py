@@ -191,3 +191,21 @@ def test_progress_line_matches_oracle_format(ctx, precision):
     # the fields parse back to the returned residuals
     f = dict(tok.split("=") for tok in st["line"].split())
     assert float(f["maxres"]) == res.max()
+
+
+def test_polfed_argument_errors(ctx):
+    """Argument checks of the C ABI come back as ValueError with the library's
+    message, before any device work (polfed.jl:1128-1134 takes nev, degree
+    and s as given; the library rejects values it cannot run)."""
+    import sparse_matrix_algorithms_amd as sm
+    A = lap1d(30)
+    for kw, msg in [(dict(nev=0, degree=10, s=2), "nev must be positive"),
+                    (dict(nev=2, degree=-1, s=2), "degree must be nonnegative"),
+                    (dict(nev=2, degree=10, s=0), "block size s must be positive"),
+                    (dict(nev=2, degree=10, s=31), "must not exceed n"),
+                    (dict(nev=2, degree=10, s=2, smin=4.0, smax=0.0), "need smin < smax")]:
+        with pytest.raises(ValueError, match=msg):
+            sm.polfed(A, kw.pop("nev"), kw.pop("degree"), ctx=ctx, **kw)
+    op = sm.MatrixFreeRealSymOp(30, None, lambda Y, X: Y.__setitem__(slice(None), A @ X))
+    with pytest.raises(ValueError, match="FP64 only"):
+        sm.polfed(op, 2, 10, s=2, smin=0.0, smax=4.0, precision="fp32", ctx=ctx)
