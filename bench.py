@@ -59,6 +59,10 @@ def parse():
                     help="slices: also run the first N slices one polfed call at a time and "
                          "as a batch of N")
     ap.add_argument("--eager", action="store_true", help="slices: launch without HIP graphs")
+    ap.add_argument("--slice-steps", type=float, default=20.0,
+                    help="slices: block width = the power of two nearest 2.8 nev / STEPS")
+    ap.add_argument("--slice-width-range", default="8,64",
+                    help="slices: smallest and largest block width, 'lo,hi'")
     ap.add_argument("--slice-learned", action="store_true",
                     help="slices: learned filter parameters per slice instead of the plain rule")
     ap.add_argument("--no-retry", action="store_true",
@@ -455,11 +459,11 @@ def _ctx_matrix(A):
 SLICE_NEV = [50, 400, 100, 300, 75, 250, 150, 350, 60, 200, 125, 275, 90, 325, 175, 225]
 
 
-def slice_width(nev):
-    """Block width that keeps about 20 block steps at mmax = 2.8 nev / s (block
-    Lanczos with fewer steps converges poorly): the power of two nearest
-    2.8 nev / 20, within [8, 64]."""
-    return int(min(64, max(8, 2 ** round(math.log2(2.8 * nev / 20)))))
+def slice_width(nev, steps=20.0, lo=8, hi=64):
+    """Block width that keeps about `steps` block steps at mmax = 2.8 nev / s
+    (block Lanczos with fewer steps converges poorly): the power of two nearest
+    2.8 nev / steps, within [lo, hi]."""
+    return int(min(hi, max(lo, 2 ** round(math.log2(2.8 * nev / steps)))))
 
 
 def bench_slices(args):
@@ -490,7 +494,8 @@ def bench_slices(args):
         c = -4.0 + 8.0 * (i + 0.5) / ns
         nev = SLICE_NEV[i % len(SLICE_NEV)]
         a, b = kpm.interval_for_count(mu, n, args.fill * nev, smin, smax, c)
-        base.append(dict(emin=a, emax=b, sigma=c, nev=nev, s=slice_width(nev)))
+        lo, hi = (int(v) for v in args.slice_width_range.split(","))
+        base.append(dict(emin=a, emax=b, sigma=c, nev=nev, s=slice_width(nev, args.slice_steps, lo, hi)))
     kw = dict(smin=smin, smax=smax, tol=1e-10 * anorm, precision=prec, ctx=ctx, want_X=False,
               refine_guard=args.refine_guard, refine_degree=args.refine_degree,
               refine_maxiter=args.refine_maxiter,
