@@ -148,6 +148,20 @@ int smalgo_csr_upload(smalgo_ctx* ctx, int64_t n, const int64_t* ptr, const int6
     auto* m = new smalgo_mat();
     m->ctx = ctx;
     m->id = g_mat_ids++;
+    {
+      // FNV-1a over n, the row pointers and the column indices
+      uint64_t h = 1469598103934665603ull;
+      auto mix = [&h](uint64_t v) {
+        for (int k = 0; k < 8; ++k) {
+          h ^= (v >> (8 * k)) & 0xff;
+          h *= 1099511628211ull;
+        }
+      };
+      mix((uint64_t)n);
+      for (int64_t i = 0; i <= n; ++i) mix((uint64_t)(uint32_t)rp[i]);
+      for (int64_t q = 0; q < nnz; ++q) mix((uint64_t)(uint32_t)ci[q]);
+      m->structure = (long long)h;
+    }
     m->A.n = (int)n;
     m->A.nnz = (int)nnz;
     // Gershgorin bounds on the host copy (polfed.jl:254-266)
@@ -353,7 +367,7 @@ int smalgo_apply_poly_block_noalloc(smalgo_ctx* ctx, const smalgo_mat* A, int64_
     Op op;
     op.A = &A->A;
     op.n = (int)n;
-    op.mat_id = A->id;
+    op.mat_id = A->structure;
     apply_filter_noalloc(*ctx, op, (int)b, dQ, (int)ldq, coeffs, K, smin, smax, dY, (int)ldy, scr);
     SMALGO_HIP(hipStreamSynchronize(ctx->stream));
   });
@@ -366,7 +380,7 @@ int smalgo_apply_poly_block(smalgo_ctx* ctx, const smalgo_mat* A, int64_t b, con
   Op op;
   op.A = &A->A;
   op.n = A->A.n;
-  op.mat_id = A->id;
+  op.mat_id = A->structure;
   return apply_poly_common(ctx, op, b, Q, ldq, coeffs, K, smin, smax, precision, Y, ldy);
 }
 
@@ -471,7 +485,7 @@ int smalgo_kpm_moments(smalgo_ctx* ctx, const smalgo_mat* A, int nmom, int R, ui
     Op op;
     op.A = &A->A;
     op.n = A->A.n;
-    op.mat_id = A->id;
+    op.mat_id = A->structure;
     kpm_moments(*ctx, op, nmom, R, seed, smin, smax, mu);
   });
 }
@@ -601,7 +615,7 @@ static bool make_op(const smalgo_mat* A, smalgo_matblock_fn fn, void* user, int6
   if (A) {
     op.A = &A->A;
     op.n = A->A.n;
-    op.mat_id = A->id;
+    op.mat_id = A->structure;
     if (n >= 0 && n != A->A.n) return false;
   } else {
     if (n <= 0 || n >= (int64_t)std::numeric_limits<int>::max()) return false;

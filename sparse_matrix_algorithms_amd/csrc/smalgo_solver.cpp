@@ -344,13 +344,27 @@ static int tile_alignment_ok(int v, const T* Q, int ldq, const T* buf0, const T*
   return 1;
 }
 
-// key of the autotuner's cache: precision, width, matrix and operand alignment
+// A 64-bit key of the autotuner's cache from its parts (splitmix64 rounds).
+static long long tune_key_mix(std::initializer_list<long long> parts) {
+  uint64_t h = 0x9E3779B97F4A7C15ull;
+  for (long long v : parts) {
+    uint64_t z = h ^ (uint64_t)v;
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    h = z ^ (z >> 31);
+  }
+  return (long long)h;
+}
+
+// key of the autotuner's cache: precision, width, the matrix's structure
+// fingerprint and operand alignment
 template <typename T>
 static long long tile_key(const Op& op, int b, const T* Q, int ldq, const T* buf0, const T* buf1,
                           int ldb, const T* Y, int ldy) {
   auto al = [&](int v) { return tile_alignment_ok<T>(v, Q, ldq, buf0, buf1, ldb, Y, ldy); };
-  return ((long long)sizeof(T) << 56) ^ ((long long)b << 32) ^ ((long long)op.mat_id << 4) ^
-         (al(8) << 3) ^ (al(4) << 2) ^ (al(2) << 1);
+  return tune_key_mix({(long long)sizeof(T), (long long)b, op.mat_id, (long long)al(8),
+                       (long long)al(4), (long long)al(2), 0});
 }
 
 // Every tile the autotuners consider at block width b: lane vectors of 1, 2,
@@ -1406,10 +1420,13 @@ struct SliceFilter {
   // Autotuner for the per-column kernel at width w: time a few steps of
   // every (vec, lanes per row, rows per team) tile on the real matrix.
   FilterTile tune(int w, float* ms_out) {
-    const long long key = ((long long)sizeof(T) << 56) ^ ((long long)w << 32) ^
-                          ((long long)op.mat_id << 4) ^ (1LL << 3);
+    const long long key = tune_key_mix({(long long)sizeof(T), (long long)w, op.mat_id, 1});
+    // a tile forced through the environment is timed but neither read from
+    // nor written to the cache, which matrices of the same structure share
+    FilterTile forced{};
+    const bool is_forced = env_tile("SMALGO_SLICE_TILE", &forced) && w % forced.vec == 0;
     auto it = cx.tuned.find(key);
-    if (it != cx.tuned.end() && cx.tuned_ms.count(key)) {
+    if (!is_forced && it != cx.tuned.end() && cx.tuned_ms.count(key)) {
       *ms_out = cx.tuned_ms[key];
       return it->second;
     }
@@ -1418,8 +1435,7 @@ struct SliceFilter {
     fill_block<T>(cx.stream, n, w, P0.p, w, T(0));
     fill_block<T>(cx.stream, n, w, P1.p, w, T(0));
     std::vector<FilterTile> cand;
-    FilterTile forced{};
-    if (env_tile("SMALGO_SLICE_TILE", &forced) && w % forced.vec == 0) {
+    if (is_forced) {
       cand.push_back(forced);
     } else if (!cx.autotune) {
       const FilterTile d = choose_tile(sizeof(T), w, {P0.p, P1.p}, {w});
@@ -1443,8 +1459,10 @@ struct SliceFilter {
             " w=" + std::to_string(w));
     const FilterTile best = cand[ib];
     const float best_ms = tmin[ib];
-    cx.tuned[key] = best;
-    cx.tuned_ms[key] = best_ms;
+    if (!is_forced) {
+      cx.tuned[key] = best;
+      cx.tuned_ms[key] = best_ms;
+    }
     *ms_out = best_ms;
     return best;
   }

@@ -80,7 +80,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   rocblas_handle blas = nullptr;
   std::string err;
-  std::map<long long, FilterTile> tuned;  // (precision, b, matrix id, alignment) -> tile
+  std::map<long long, FilterTile> tuned;  // (precision, width, structure, alignment) -> tile
   bool autotune = true;
   FilterTile last_tile{};  // tile of the most recent filter
   int last_launches_per_step = 1;  // kernel launches per filter step (panels)
@@ -93,6 +93,10 @@ struct Mat {
   DevCSR A;
   int symmetric = -1;  // computed at upload: 1 / 0
   long long id = 0;
+  // fingerprint of the sparsity structure (n, row pointers, column indices):
+  // a filter tile's speed depends on the structure only, so matrices with the
+  // same structure share the autotuner's picks
+  long long structure = 0;
 };
 
 // Operator: device CSR or host callback (Y <- A X on column-major host arrays)
@@ -101,7 +105,7 @@ struct Op {
   smalgo_matblock_fn fn = nullptr;
   void* user = nullptr;
   int n = 0;
-  long long mat_id = 0;
+  long long mat_id = 0;  // the matrix's structure fingerprint (tuning cache key)
   bool is_callback() const { return fn != nullptr; }
 };
 
