@@ -370,10 +370,12 @@ static long long tile_key(const Op& op, int b, const T* Q, int ldq, const T* buf
 // Every tile the autotuners consider at block width b: lane vectors of 1, 2,
 // 4 or 8 elements that vec_ok admits (operand alignment, width), column
 // chunks lpr·vec of at most twice the width and at most 2 KB, 1, 2 or 4 rows
-// per team, and the three CSR forms (per lane, staged in LDS, staged with
-// own-row prefetch; the last for 1 or 2 rows per team).
+// per team, and the CSR forms: per lane, staged in LDS, and (when `prefetch`)
+// staged with own-row prefetch for 1 or 2 rows per team.  The prefetch form
+// has only won where a step's working set exceeds the Infinity Cache, so the
+// callers offer it there only.
 template <typename VecOk>
-static std::vector<FilterTile> tile_candidates(int b, size_t elem, VecOk&& vec_ok) {
+static std::vector<FilterTile> tile_candidates(int b, size_t elem, bool prefetch, VecOk&& vec_ok) {
   std::vector<FilterTile> cand;
   for (int v : {1, 2, 4, 8}) {
     if (!vec_ok(v)) continue;
@@ -382,7 +384,7 @@ static std::vector<FilterTile> tile_candidates(int b, size_t elem, VecOk&& vec_o
       if (w > 2 * std::max(b, 4) || w * (int)elem > 2048) continue;
       for (int rt : {1, 2, 4})
         for (int lds : {0, 1, 2})
-          if (lds < 2 || rt <= 2) cand.push_back(FilterTile{v, l, rt, lds});
+          if (lds < 2 || (prefetch && rt <= 2)) cand.push_back(FilterTile{v, l, rt, lds});
     }
   }
   return cand;
@@ -431,8 +433,10 @@ static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ld
   const long long key = tile_key<T>(op, b, Q, ldq, buf0, buf1, ldb, Y, ldy);
   auto it = cx.tuned.find(key);
   if (it != cx.tuned.end()) return it->second;
+  // working set of one step beyond the 240 MB Infinity-Cache budget?
+  const bool beyond = 3.0 * op.n * b * sizeof(T) + (double)op.A->nnz * (sizeof(T) + 4) > 240e6;
   const std::vector<FilterTile> cand =
-      tile_candidates(b, sizeof(T), [&](int v) { return al(v) && (v == 1 || b >= v); });
+      tile_candidates(b, sizeof(T), beyond, [&](int v) { return al(v) && (v == 1 || b >= v); });
   if (cand.empty()) return def;
   // The steps timed are the ones the recurrence runs from its third step on:
   // Q, X1 = buf0 and X2 = buf1 distinct, the output written in place over X2
@@ -1441,7 +1445,8 @@ struct SliceFilter {
       const FilterTile d = choose_tile(sizeof(T), w, {P0.p, P1.p}, {w});
       cand.push_back(FilterTile{d.vec, d.lpr, 1});
     } else {
-      cand = tile_candidates(w, sizeof(T), [&](int v) { return w % v == 0; });
+      // panels are sized to stay cache-resident: no prefetch form
+      cand = tile_candidates(w, sizeof(T), false, [&](int v) { return w % v == 0; });
     }
     // The step the recurrence runs: Q, X1 and X2 distinct, the output in
     // place over X2 (out = cq∘Q + a1 A X1 + b1 X1 - X2); two sweeps of 4 steps.
