@@ -86,7 +86,7 @@ def parse():
     ap.add_argument("--refine-maxiter", type=int, default=3)
     ap.add_argument("--refine-guard-degree", type=int, default=None,
                     help="guard vectors' filter degree in the first refinement iteration "
-                         "(default: a tenth of the refinement degree)")
+                         "(default: a fiftieth of the refinement degree)")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

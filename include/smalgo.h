@@ -235,8 +235,8 @@ typedef struct smalgo_polfed_params {
   double refine_tol;     /* lock a pair when ‖r‖ < refine_tol·‖A‖; default -1: when ‖r‖ < tol,
                             the caller's absolute tolerance of the return rule */
   int refine_guard_degree; /* degree of the guard vectors' filter in the first refinement
-                              iteration (default -1: a tenth of the refinement degree; the
-                              guards only widen the Rayleigh–Ritz subspace) */
+                              iteration (default -1: a fiftieth of the refinement degree,
+                              rounded up; the guards only widen the Rayleigh–Ritz subspace) */
 } smalgo_polfed_params;
 
 void smalgo_polfed_default_params(smalgo_polfed_params* p);

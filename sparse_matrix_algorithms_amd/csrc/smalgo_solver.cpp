@@ -1997,11 +1997,11 @@ static Result* polfed_extract(Ctx& cx, Dense& dn, PhaseTimer& tm, const Op& op, 
         const double tol_abs = p.refine_tol >= 0 ? p.refine_tol * anorm : p.tol;
         // The guard vectors only widen the Rayleigh–Ritz subspace by directions just
         // outside the window, which the FP32 basis already carries; in the first
-        // iteration they take a window of refine_guard_degree (default Kr/10).
+        // iteration they take a window of refine_guard_degree (default Kr/50).
         std::vector<double> cg;
         {
           const int Kg = p.refine_guard_degree >= 0 ? p.refine_guard_degree
-                                                    : std::max(1, (Kr + 9) / 10);
+                                                    : std::max(1, (Kr + 49) / 50);
           if (Kg < Kr)
             cg = build_coeffs(Kg, smin, smax, p.sigma, p.emin, p.emax, p.window_type,
                               p.jackson != 0);

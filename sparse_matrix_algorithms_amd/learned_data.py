@@ -42,13 +42,21 @@ def instance(i: int, base: int = 2026):
                 target=int(rng.integers(40, 241)))
 
 
+# mixed-path runs: guard vectors' refinement degree as a fraction of the
+# degree (None: the library default, a fiftieth); set by --guard-fraction
+GUARD_FRACTION = None
+
+
 def _solve(sm, dm, nev, K, mmax, emin, emax, smin, smax, anorm, ctx, precision="fp64",
            refine_degree=None):
     t = time.perf_counter()
+    gd = None
+    if precision == "fp32" and GUARD_FRACTION is not None:
+        gd = max(1, int(math.ceil(GUARD_FRACTION * (refine_degree or K))))
     lam, _, res, st = sm.polfed(dm, nev, K, s=S, smin=smin, smax=smax, emin=emin, emax=emax,
                                 sigma=0.5 * (emin + emax), mmax=mmax, tol=1e-10 * anorm, ctx=ctx,
                                 return_stats=True, want_X=False, precision=precision,
-                                refine_degree=refine_degree)
+                                refine_degree=refine_degree, refine_guard_degree=gd)
     dt = time.perf_counter() - t
     ok = (lam >= emin) & (lam <= emax) & (res <= 1e-10 * anorm)
     return np.sort(lam[ok]), dt, st
@@ -336,9 +344,14 @@ def main():
                          "held-out comparison (--const: the chosen constant rules)")
     ap.add_argument("--const", action="append", default=[],
                     help="rules --set heldout: a constant rule 'kappa,factor' (repeatable)")
+    ap.add_argument("--guard-fraction", type=float, default=None,
+                    help="mixed-path runs: guard vectors' refinement degree as this fraction "
+                         "of the degree (default: the library's, a fiftieth)")
     ap.add_argument("--redo-unconverged", type=Path, default=None,
                     help="regenerate the instances of this log whose plain run did not converge")
     a = ap.parse_args()
+    global GUARD_FRACTION
+    GUARD_FRACTION = a.guard_fraction
     if a.cmd == "gen":
         idx = None
         if a.redo_unconverged:

@@ -134,7 +134,7 @@ def test_mixed_refine_residual_form_matches_fp64_filter(ctx, monkeypatch):
 @pytest.mark.parametrize("guard_degree", [None, 1, 700])
 def test_mixed_refine_guard_degree(ctx, guard_degree):
     """The guard vectors of the first refinement iteration only widen the
-    Rayleigh–Ritz subspace: at the default (a tenth of the refinement degree),
+    Rayleigh–Ritz subspace: at the default (a fiftieth of the refinement degree),
     at degree 1 and at the full degree the mixed path returns the dense-eigh
     eigenvalues, and the lower guard degree runs fewer filter steps."""
     import sparse_matrix_algorithms_amd as sm

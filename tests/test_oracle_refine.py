@@ -72,12 +72,12 @@ def _rr(A, Y):
     return w, X, np.linalg.norm(A @ X - X * w, axis=0)
 
 
-@pytest.mark.parametrize("guard_scale", [1.0, 0.1, 0.0])
+@pytest.mark.parametrize("guard_scale", [1.0, 0.1, 0.02, 0.0])
 def test_low_degree_guards_keep_the_refined_pairs_exact(guard_scale):
     """The design claim behind the guard-degree default of the mixed path:
     the guards only widen the Rayleigh–Ritz subspace by directions just
-    outside the window, so filtering them at a tenth of the candidates'
-    degree (or not at all: degree 0) leaves the refined pairs of the window at
+    outside the window, so filtering them at a tenth or a fiftieth of the
+    candidates' degree (or not at all: degree 0) leaves the refined pairs of the window at
     FP64 accuracy.  Candidates and guards carry FP32-sized noise spread over
     the whole spectrum plus a 1e-4 admixture of their neighbours, as Ritz
     vectors of an FP32-built basis do; the candidates get the full-degree
