@@ -502,6 +502,13 @@ static void clenshaw_recurrence(const double* co, int K, double smin, double sma
 // panels (201 MB) run at 5.7 TB/s, 512-byte panels (403 MB) at 3.8 TB/s.
 template <typename T>
 static int cache_panel_width(const Op& op) {
+  // SMALGO_PANEL_BYTES=128|256|512 fixes the row-segment width (tuning override)
+  static const int forced = [] {
+    const char* e = std::getenv("SMALGO_PANEL_BYTES");
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 128 || v == 256 || v == 512) ? v : 0;
+  }();
+  if (forced) return forced / (int)sizeof(T);
   const double mat = op.A ? (double)op.A->nnz * (sizeof(T) + 4) : 0.0;
   for (int bytes = 512; bytes > 128; bytes /= 2)
     if (3.0 * op.n * bytes + mat <= 240e6) return bytes / (int)sizeof(T);
