@@ -219,6 +219,8 @@ def main():
                     refine_iterations=st_.get("refine_iterations", 0),
                     t_refine_host=st_.get("t_refine_host", 0.0),
                     locked=st_.get("locked", 0),
+                    tile=",".join(str(int(st_.get(k, 0))) for k in
+                                  ("tile_vec", "tile_lpr", "tile_rt", "tile_lds", "tile_split")),
                     filter_gbs=round(st_["spmm_bytes"] / st_["t_filter"] / 1e9, 1),
                     orth_gemm_tflops=round(tf, 3),
                     # MFMA use of the block-orthogonalisation phase (Gram and update GEMMs

@@ -57,6 +57,8 @@ struct FilterTile {
   int rt = 1;    // consecutive rows per lane team
   int lds = 0;   // 1: the workgroup's CSR tile is staged in LDS first; 2: staged, with the
                  // own-row loads issued before the staging barrier (rt <= 2)
+  int split = 1; // 2: each step runs as two launches over the two column halves, one on each
+                 // of the context's two streams (vec/lpr/rt/lds then describe a half)
 };
 
 template <typename T>

@@ -278,6 +278,131 @@ void apply_A(Ctx& cx, const Op& op, int b, const T* X, int ldx, T* Y, int ldy) {
 template void apply_A<double>(Ctx&, const Op&, int, const double*, int, double*, int);
 template void apply_A<float>(Ctx&, const Op&, int, const float*, int, float*, int);
 
+// ---------------------------------------------------------------------------
+// Split filter steps (FilterTile::split = 2): a step of width b runs as two
+// launches over the column halves [0, h) and [h, b), the first on the
+// context's stream and the second on its side stream.  The halves share no
+// column, so their recurrences are independent; run concurrently, each chain
+// fills the other's launch ramp and drain.  h is b/2 rounded down to whole
+// 128-byte lines, and the split is made only when every block pointer and
+// leading dimension keeps the halves on separate cache lines.
+// ---------------------------------------------------------------------------
+static int split_half(int b, size_t elem) {
+  const int g = (int)(128 / elem);
+  const int h = (b / 2) / g * g;
+  return (h >= g && b - h >= g) ? h : 0;
+}
+
+static bool split_ok(int b, size_t elem, std::initializer_list<const void*> ptrs,
+                     std::initializer_list<int> lds) {
+  if (split_half(b, elem) == 0) return false;
+  for (const void* q : ptrs)
+    if (q && reinterpret_cast<uintptr_t>(q) % 128) return false;
+  for (int ld : lds)
+    if (ld && ((size_t)ld * elem) % 128) return false;
+  return true;
+}
+
+static void side_stream(Ctx& cx) {
+  if (cx.side) return;
+  SMALGO_HIP(hipEventCreateWithFlags(&cx.fork_ev, hipEventDisableTiming));
+  SMALGO_HIP(hipEventCreateWithFlags(&cx.join_ev, hipEventDisableTiming));
+  SMALGO_HIP(hipStreamCreateWithFlags(&cx.side, hipStreamNonBlocking));
+}
+
+// the side stream waits for all work issued so far on the main stream
+static void split_fork(Ctx& cx) {
+  side_stream(cx);
+  SMALGO_HIP(hipEventRecord(cx.fork_ev, cx.stream));
+  SMALGO_HIP(hipStreamWaitEvent(cx.side, cx.fork_ev, 0));
+}
+
+// the main stream waits for all work issued so far on the side stream
+static void split_join(Ctx& cx) {
+  SMALGO_HIP(hipEventRecord(cx.join_ev, cx.side));
+  SMALGO_HIP(hipStreamWaitEvent(cx.stream, cx.join_ev, 0));
+}
+
+// Fork on entry when `on`; join on exit, also when a launch throws.  Split
+// launches happen only inside such a scope (an inner scope adds nothing).
+struct SplitScope {
+  Ctx& cx;
+  bool on;
+  SplitScope(Ctx& c, bool o) : cx(c), on(o && !c.split_active) {
+    if (on) {
+      split_fork(cx);
+      cx.split_active = true;
+      cx.split_h = -1;
+    }
+  }
+  ~SplitScope() {
+    if (on) {
+      cx.split_active = false;
+      (void)hipEventRecord(cx.join_ev, cx.side);
+      (void)hipStreamWaitEvent(cx.stream, cx.join_ev, 0);
+    }
+  }
+};
+
+// Column split of the next launch (0: one launch on the main stream).  When it
+// differs from the last launch's, the streams meet first (join, then fork):
+// the buffers the recurrence reuses (ping-pong blocks, panels) change hands
+// between the streams at the new column boundary.
+static int split_begin(Ctx& cx, int h) {
+  if (!cx.split_active) return 0;
+  if (cx.split_h >= 0 && h != cx.split_h) {
+    split_join(cx);
+    split_fork(cx);
+  }
+  cx.split_h = h;
+  return h;
+}
+
+// One fused step with the tile's launch shape: one launch, or (split = 2)
+// one per column half on the two streams, inside a SplitScope of the caller.
+// A split tile outside a scope, or on operands that do not allow the split,
+// runs as one launch.
+template <typename T>
+static void launch_step(Ctx& cx, const DevCSR& A, int b, const T* Q, int ldq, T cq, const T* X1,
+                        int ld1, T a1, T b1, const T* X2, int ld2, T s2, T* out, int ldo,
+                        FilterTile t) {
+  const int h = split_begin(
+      cx, t.split == 2 && split_ok(b, sizeof(T), {Q, X1, X2, out}, {ldq, ld1, ld2, ldo})
+              ? split_half(b, sizeof(T))
+              : 0);
+  if (h == 0) {
+    launch_clenshaw_step<T>(cx.stream, A, b, Q, ldq, cq, X1, ld1, a1, b1, X2, ld2, s2, out, ldo, t);
+    return;
+  }
+  auto off = [h](const T* q) { return q ? q + h : nullptr; };
+  ++cx.split_steps;
+  launch_clenshaw_step<T>(cx.stream, A, h, Q, ldq, cq, X1, ld1, a1, b1, X2, ld2, s2, out, ldo, t);
+  launch_clenshaw_step<T>(cx.side, A, b - h, off(Q), ldq, cq, off(X1), ld1, a1, b1, off(X2), ld2,
+                          s2, out + h, ldo, t);
+}
+
+// the same for per-column Q coefficients (row cq of b values)
+template <typename T>
+static void launch_step_cols(Ctx& cx, const DevCSR& A, int b, const T* Q, int ldq, const T* cq,
+                             const T* X1, int ld1, T a1, T b1, const T* X2, int ld2, T s2, T* out,
+                             int ldo, FilterTile t) {
+  const int h = split_begin(
+      cx, t.split == 2 && split_ok(b, sizeof(T), {Q, X1, X2, out}, {ldq, ld1, ld2, ldo})
+              ? split_half(b, sizeof(T))
+              : 0);
+  if (h == 0) {
+    launch_clenshaw_step_cols<T>(cx.stream, A, b, Q, ldq, cq, X1, ld1, a1, b1, X2, ld2, s2, out,
+                                 ldo, t);
+    return;
+  }
+  auto off = [h](const T* q) { return q ? q + h : nullptr; };
+  ++cx.split_steps;
+  launch_clenshaw_step_cols<T>(cx.stream, A, h, Q, ldq, cq, X1, ld1, a1, b1, X2, ld2, s2, out,
+                               ldo, t);
+  launch_clenshaw_step_cols<T>(cx.side, A, b - h, off(Q), ldq, cq + h, off(X1), ld1, a1, b1,
+                               off(X2), ld2, s2, out + h, ldo, t);
+}
+
 // One Clenshaw step out = cq*Q + a1*(A X1) + b1*X1 - s2*X2 with either
 // operator kind.  X1/X2 are given as (pointer, implicit multiple of Q).
 template <typename T>
@@ -326,8 +451,8 @@ static void filter_step(Ctx& cx, const Op& op, int b, const T* Q, int ldq, doubl
     }
   }
   FilterTile t = tile ? *tile : choose_tile(sizeof(T), b, {Q, x1p, x2p, out}, {ldq, ld1, ldb, ldo});
-  launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, (T)cqe, x1p, ld1, (T)a1e, (T)b1e, x2p, ldb,
-                          (T)s2e, out, ldo, t);
+  launch_step<T>(cx, *op.A, b, Q, ldq, (T)cqe, x1p, ld1, (T)a1e, (T)b1e, x2p, ldb, (T)s2e, out, ldo,
+                 t);
 }
 
 // Autotuner: on the first filter of a (precision, width, matrix, alignment)
@@ -363,8 +488,9 @@ template <typename T>
 static long long tile_key(const Op& op, int b, const T* Q, int ldq, const T* buf0, const T* buf1,
                           int ldb, const T* Y, int ldy) {
   auto al = [&](int v) { return tile_alignment_ok<T>(v, Q, ldq, buf0, buf1, ldb, Y, ldy); };
+  const bool sp = split_ok(b, sizeof(T), {Q, buf0, buf1, Y}, {ldq, ldb, ldy});
   return tune_key_mix({(long long)sizeof(T), (long long)b, op.mat_id, (long long)al(8),
-                       (long long)al(4), (long long)al(2), 0});
+                       (long long)al(4), (long long)al(2), (long long)sp, 0});
 }
 
 // Every tile the autotuners consider at block width b: lane vectors of 1, 2,
@@ -373,20 +499,30 @@ static long long tile_key(const Op& op, int b, const T* Q, int ldq, const T* buf
 // per team, and the CSR forms: per lane, staged in LDS, and (when `prefetch`)
 // staged with own-row prefetch for 1 or 2 rows per team.  The prefetch form
 // has only won where a step's working set exceeds the Infinity Cache, so the
-// callers offer it there only.
+// callers offer it there only.  With `split`, the split tiles follow: the
+// same shapes at the half width (split_half), with 1 or 2 rows per team and
+// without the prefetch form.
 template <typename VecOk>
-static std::vector<FilterTile> tile_candidates(int b, size_t elem, bool prefetch, VecOk&& vec_ok) {
+static std::vector<FilterTile> tile_candidates(int b, size_t elem, bool prefetch, bool split,
+                                               VecOk&& vec_ok) {
   std::vector<FilterTile> cand;
-  for (int v : {1, 2, 4, 8}) {
-    if (!vec_ok(v)) continue;
-    for (int l : {4, 8, 16, 32, 64}) {
-      const int w = l * v;  // chunk width in elements
-      if (w > 2 * std::max(b, 4) || w * (int)elem > 2048) continue;
-      for (int rt : {1, 2, 4})
-        for (int lds : {0, 1, 2})
-          if (lds < 2 || (prefetch && rt <= 2)) cand.push_back(FilterTile{v, l, rt, lds});
+  auto add = [&](int w0, int sp) {
+    for (int v : {1, 2, 4, 8}) {
+      if (!vec_ok(v) || w0 < v) continue;
+      for (int l : {4, 8, 16, 32, 64}) {
+        const int w = l * v;  // chunk width in elements
+        if (w > 2 * std::max(w0, 4) || w * (int)elem > 2048) continue;
+        for (int rt : {1, 2, 4})
+          for (int lds : {0, 1, 2}) {
+            if (sp == 2 && (rt > 2 || lds == 2)) continue;
+            if (lds < 2 || (prefetch && rt <= 2)) cand.push_back(FilterTile{v, l, rt, lds, sp});
+          }
+      }
     }
-  }
+  };
+  add(b, 1);
+  const int h = split ? split_half(b, elem) : 0;
+  if (h > 0) add(h, 2);
   return cand;
 }
 
@@ -397,17 +533,34 @@ static std::vector<FilterTile> tile_candidates(int b, size_t elem, bool prefetch
 template <typename StepFn>
 static size_t time_tiles(Ctx& cx, const std::vector<FilterTile>& cand, int sweeps, int reps,
                          StepFn&& step, std::vector<float>& tmin, const std::string& what) {
-  Event e0, e1;
+  Event e0, e1, e1s;
   tmin.assign(cand.size(), 1e30f);
   for (int sweep = 0; sweep < sweeps; ++sweep)
     for (size_t i = 0; i < cand.size(); ++i) {
-      step(cand[i]);
-      SMALGO_HIP(hipEventRecord(e0, cx.stream));
-      for (int r = 0; r < reps; ++r) step(cand[i]);
-      SMALGO_HIP(hipEventRecord(e1, cx.stream));
+      // A split tile's steps run on both streams inside one fork/join pair,
+      // as in the recurrence.  Their time runs from e0 on the main stream to
+      // the later of the two streams' end events, so the fork and the join
+      // (paid once per filter, not per step) stay outside it.
+      const bool sp = cand[i].split == 2;
+      bool two = false;
+      {
+        SplitScope scope(cx, sp);
+        step(cand[i]);  // warm
+        SMALGO_HIP(hipEventRecord(e0, cx.stream));
+        for (int r = 0; r < reps; ++r) step(cand[i]);
+        SMALGO_HIP(hipEventRecord(e1, cx.stream));
+        two = sp && cx.split_active;
+        if (two) SMALGO_HIP(hipEventRecord(e1s, cx.side));
+      }
       SMALGO_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
       SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (two) {
+        float ms2 = 0.f;
+        SMALGO_HIP(hipEventSynchronize(e1s));
+        SMALGO_HIP(hipEventElapsedTime(&ms2, e0, e1s));
+        ms = std::max(ms, ms2);
+      }
       tmin[i] = std::min(tmin[i], ms / (float)reps);
     }
   size_t best = 0;
@@ -416,10 +569,12 @@ static size_t time_tiles(Ctx& cx, const std::vector<FilterTile>& cand, int sweep
   const char* lg = std::getenv("SMALGO_TUNE_LOG");
   if (lg && lg[0] == '1') {
     for (size_t i = 0; i < cand.size(); ++i)
-      std::fprintf(stderr, "[smalgo tune] %s vec=%d lpr=%d rt=%d lds=%d: %.2f us\n", what.c_str(),
-                   cand[i].vec, cand[i].lpr, cand[i].rt, cand[i].lds, tmin[i] * 1e3f);
-    std::fprintf(stderr, "[smalgo tune] %s picked vec=%d lpr=%d rt=%d lds=%d\n", what.c_str(),
-                 cand[best].vec, cand[best].lpr, cand[best].rt, cand[best].lds);
+      std::fprintf(stderr, "[smalgo tune] %s vec=%d lpr=%d rt=%d lds=%d split=%d: %.2f us\n",
+                   what.c_str(), cand[i].vec, cand[i].lpr, cand[i].rt, cand[i].lds, cand[i].split,
+                   tmin[i] * 1e3f);
+    std::fprintf(stderr, "[smalgo tune] %s picked vec=%d lpr=%d rt=%d lds=%d split=%d\n",
+                 what.c_str(), cand[best].vec, cand[best].lpr, cand[best].rt, cand[best].lds,
+                 cand[best].split);
   }
   return best;
 }
@@ -435,8 +590,9 @@ static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ld
   if (it != cx.tuned.end()) return it->second;
   // working set of one step beyond the 240 MB Infinity-Cache budget?
   const bool beyond = 3.0 * op.n * b * sizeof(T) + (double)op.A->nnz * (sizeof(T) + 4) > 240e6;
+  const bool sp = split_ok(b, sizeof(T), {Q, buf0, buf1, Y}, {ldq, ldb, ldy});
   const std::vector<FilterTile> cand =
-      tile_candidates(b, sizeof(T), beyond, [&](int v) { return al(v) && (v == 1 || b >= v); });
+      tile_candidates(b, sizeof(T), beyond, sp, [&](int v) { return al(v); });
   if (cand.empty()) return def;
   // The steps timed are the ones the recurrence runs from its third step on:
   // Q, X1 = buf0 and X2 = buf1 distinct, the output written in place over X2
@@ -451,8 +607,8 @@ static FilterTile autotune_tile(Ctx& cx, const Op& op, int b, const T* Q, int ld
   const size_t ib = time_tiles(
       cx, cand, 2, 6,
       [&](const FilterTile& t) {
-        launch_clenshaw_step<T>(cx.stream, *op.A, b, Q, ldq, T(0.5), buf0, ldb, T(0.1), T(0.1),
-                                buf1, ldb, T(1), buf1, ldb, t);
+        launch_step<T>(cx, *op.A, b, Q, ldq, T(0.5), buf0, ldb, T(0.1), T(0.1), buf1, ldb, T(1),
+                       buf1, ldb, t);
       },
       tmin,
       std::string(sizeof(T) == 4 ? "fp32" : "fp64") + " n=" + std::to_string(op.n) +
@@ -515,16 +671,15 @@ static int cache_panel_width(const Op& op) {
   return 128 / (int)sizeof(T);
 }
 
-// Tuning overrides (environment): SMALGO_FILTER_TILE="vec,lpr,rt" fixes the
-// tile of row-major filters, SMALGO_PANEL_TILE likewise for panel filters
-// (lpr*vec must equal the panel width), SMALGO_NO_PANELS=1 keeps wide blocks
-// row-major.
+// Tuning overrides (environment): SMALGO_FILTER_TILE="vec,lpr,rt[,lds[,split]]"
+// fixes the tile of row-major filters, SMALGO_PANEL_TILE likewise for panel
+// filters, SMALGO_NO_PANELS=1 keeps wide blocks row-major.
 static bool env_tile(const char* name, FilterTile* t) {
   const char* v = std::getenv(name);
   if (!v) return false;
-  int a = 0, b = 0, c = 1, d = 0;
-  if (std::sscanf(v, "%d,%d,%d,%d", &a, &b, &c, &d) < 2) return false;
-  *t = FilterTile{a, b, c, d};
+  int a = 0, b = 0, c = 1, d = 0, e = 1;
+  if (std::sscanf(v, "%d,%d,%d,%d,%d", &a, &b, &c, &d, &e) < 2) return false;
+  *t = FilterTile{a, b, c, d, e == 2 ? 2 : 1};
   return true;
 }
 
@@ -558,32 +713,35 @@ void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::
       tile = autotune_tile<T>(cx, op, pw, Qp.p, pw, P0.p, P1.p, pw, Yp.p, pw);
     cx.last_tile = tile;
     cx.last_launches_per_step = npan;
-    for (int pn = 0; pn < npan; ++pn) {
-      const int bp = std::min(pw, b - pn * pw);
-      const T* Qn = Qp.p + (size_t)pn * ps;
-      auto step = [&](double cq, double a1, double b1, FBlk<T> X1, FBlk<T> X2, T* out) {
-        const T* x1p = X1.p;
-        double a1e = a1, b1e = b1;
-        if (!x1p) {
-          if (X1.qs == 0.0) {
-            a1e = b1e = 0.0;
-          } else {
-            x1p = Qn;
-            a1e = a1 * X1.qs;
-            b1e = b1 * X1.qs;
+    {
+      SplitScope split(cx, tile.split == 2);  // joined before the unpacking below
+      for (int pn = 0; pn < npan; ++pn) {
+        const int bp = std::min(pw, b - pn * pw);
+        const T* Qn = Qp.p + (size_t)pn * ps;
+        auto step = [&](double cq, double a1, double b1, FBlk<T> X1, FBlk<T> X2, T* out) {
+          const T* x1p = X1.p;
+          double a1e = a1, b1e = b1;
+          if (!x1p) {
+            if (X1.qs == 0.0) {
+              a1e = b1e = 0.0;
+            } else {
+              x1p = Qn;
+              a1e = a1 * X1.qs;
+              b1e = b1 * X1.qs;
+            }
           }
-        }
-        const T* x2p = X2.p;
-        double cqe = cq, s2e = 1.0;
-        if (!x2p) {
-          cqe = cq - X2.qs;
-          s2e = 0.0;
-        }
-        launch_clenshaw_step<T>(cx.stream, *op.A, bp, Qn, pw, (T)cqe, x1p, pw, (T)a1e, (T)b1e,
-                                x2p, pw, (T)s2e, out, pw, tile);
-      };
-      clenshaw_recurrence<T>(co.data(), (int)co.size() - 1, smin, smax, P0.p, P1.p,
-                             Yp.p + (size_t)pn * ps, step);
+          const T* x2p = X2.p;
+          double cqe = cq, s2e = 1.0;
+          if (!x2p) {
+            cqe = cq - X2.qs;
+            s2e = 0.0;
+          }
+          launch_step<T>(cx, *op.A, bp, Qn, pw, (T)cqe, x1p, pw, (T)a1e, (T)b1e, x2p, pw, (T)s2e,
+                         out, pw, tile);
+        };
+        clenshaw_recurrence<T>(co.data(), (int)co.size() - 1, smin, smax, P0.p, P1.p,
+                               Yp.p + (size_t)pn * ps, step);
+      }
     }
     from_panels<T>(cx.stream, n, b, pw, Yp.p, Y, ldy);
     return;
@@ -601,6 +759,7 @@ void apply_filter(Ctx& cx, const Op& op, int b, const T* Q, int ldq, const std::
     const int ldo = (out == Y) ? ldy : ldb;
     filter_step<T>(cx, op, b, Q, ldq, cq, a1, b1, X1, X2, ldb, out, ldo, pbuf, tp);
   };
+  SplitScope split(cx, tp && tile.split == 2);
   clenshaw_recurrence<T>(co.data(), (int)co.size() - 1, smin, smax, buf0, buf1, Y, step);
 }
 template void apply_filter<double>(Ctx&, const Op&, int, const double*, int,
@@ -640,6 +799,7 @@ void apply_filter_noalloc(Ctx& cx, const Op& op, int b, const double* Q, int ldq
   FilterTile tile = choose_tile(sizeof(double), b, {Q, buf0, buf1, Y}, {ldq, ldy, ldy});
   auto it = cx.tuned.find(tile_key<double>(op, b, Q, ldq, buf0, buf1, ldy, Y, ldy));
   if (it != cx.tuned.end()) tile = it->second;
+  tile.split = 1;  // one stream: a split tile's half shape runs over the whole width
   cx.last_tile = tile;
   cx.last_launches_per_step = 1;
   auto step = [&](double cq, double a1, double b1, FBlk<double> X1, FBlk<double> X2, double* out) {
@@ -1452,8 +1612,10 @@ struct SliceFilter {
       const FilterTile d = choose_tile(sizeof(T), w, {P0.p, P1.p}, {w});
       cand.push_back(FilterTile{d.vec, d.lpr, 1});
     } else {
-      // panels are sized to stay cache-resident: no prefetch form
-      cand = tile_candidates(w, sizeof(T), false, [&](int v) { return w % v == 0; });
+      // panels are sized to stay cache-resident: no prefetch form; panels
+      // (pool blocks, ld = w) allow the split when w does
+      cand = tile_candidates(w, sizeof(T), false, split_ok(w, sizeof(T), {P0.p, P1.p}, {w}),
+                             [&](int v) { return w % v == 0; });
     }
     // The step the recurrence runs: Q, X1 and X2 distinct, the output in
     // place over X2 (out = cq∘Q + a1 A X1 + b1 X1 - X2); two sweeps of 4 steps.
@@ -1463,8 +1625,8 @@ struct SliceFilter {
     const size_t ib = time_tiles(
         cx, cand, cand.size() > 1 ? 2 : 1, 4,
         [&](const FilterTile& tl) {
-          launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, qt.p, w, co.p, P0.p, w, T(0.1), T(0.1),
-                                       P1.p, w, T(1), P1.p, w, tl);
+          launch_step_cols<T>(cx, *op.A, w, qt.p, w, co.p, P0.p, w, T(0.1), T(0.1), P1.p, w, T(1),
+                              P1.p, w, tl);
         },
         tmin,
         std::string("panel ") + (sizeof(T) == 4 ? "fp32" : "fp64") + " n=" + std::to_string(n) +
@@ -1558,9 +1720,14 @@ struct SliceFilter {
         default: return nullptr;
       }
     };
-    launch_clenshaw_step_cols<T>(cx.stream, *op.A, pw, Qn, pw, tab.p + st.cq, ptr(st.x1), pw,
-                                 (T)st.a1, (T)st.b1, ptr(st.x2), pw, (T)st.s2, ptr(st.out), pw,
-                                 P.tile);
+    launch_step_cols<T>(cx, *op.A, pw, Qn, pw, tab.p + st.cq, ptr(st.x1), pw, (T)st.a1,
+                        (T)st.b1, ptr(st.x2), pw, (T)st.s2, ptr(st.out), pw, P.tile);
+  }
+
+  bool any_split() const {
+    for (const auto& P : pk.pan)
+      if (P.tile.split == 2) return true;
+    return false;
   }
 
   // Y-panels = p_i(Â) Q-panels for every slice.  With graph = true the
@@ -1568,6 +1735,7 @@ struct SliceFilter {
   // graph_nodes_max kernels (kernel nodes only) and replayed afterwards.
   void run(bool graph) {
     if (!graph) {
+      SplitScope split(cx, any_split());
       for (size_t p = 0; p < pk.pan.size(); ++p)
         for (const Step& st : steps[p]) launch((int)p, st);
       return;
@@ -1587,8 +1755,12 @@ struct SliceFilter {
     for (size_t b = 0; b < all.size(); b += graph_nodes_max) {
       const size_t e = std::min(all.size(), b + (size_t)graph_nodes_max);
       hipGraph_t g = nullptr;
+      if (any_split()) side_stream(cx);
       SMALGO_HIP(hipStreamBeginCapture(cx.stream, hipStreamCaptureModeThreadLocal));
       try {
+        // split panels: the side stream joins the capture through the fork
+        // event and leaves it through the join event (graph edges)
+        SplitScope split(cx, any_split());
         for (size_t k = b; k < e; ++k) launch(all[k].first, steps[all[k].first][all[k].second]);
       } catch (...) {
         (void)hipStreamEndCapture(cx.stream, &g);
@@ -2259,6 +2431,7 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
   fill_block<T>(cx.stream, n, B.ldV, B.V.p, B.ldV, T(0));
   basis_start<T>(cx, dn, sw.ow, B, n, Q1host, p.seed, p.rtol);
   FilterTile basis_tile{};
+  const long long split0 = cx.split_steps;
   int basis_lps = 1;
   long long filter_steps = 0;
   double gemm_flops = 0;
@@ -2307,6 +2480,8 @@ static Result* polfed_run(Ctx& cx, const Op& op, int nev, int degree, const smal
   out->stats["tile_lpr"] = basis_tile.lpr;
   out->stats["tile_rt"] = basis_tile.rt;
   out->stats["tile_lds"] = basis_tile.lds;
+  out->stats["tile_split"] = basis_tile.split;
+  out->stats["filter_split_steps"] = (double)(cx.split_steps - split0);
   out->stats["spmm_bytes"] = bytes;
   return out.release();
 }
@@ -2369,7 +2544,8 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
     default:
       throw Error(-1, "Unknown window type, possible are :step, :delta, and :smooth");
   }
-  const int ld = round_up(m0, 4);
+  // rows start on 128-byte lines, so the filter may split the block by columns
+  const int ld = round_up(m0, 16);
   PhaseTimer tm(cx.stream);
   Dense dn(cx);
   OrthWork ow;
@@ -2392,6 +2568,7 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
   int its = 0;
   long long filter_steps = 0;
   FilterTile cheb_tile{};
+  const long long split0 = cx.split_steps;
   int cheb_lps = 1;
   for (int it = 1; it <= p.maxiter; ++it) {
     its = it;
@@ -2506,6 +2683,8 @@ Result* chebyshev_solve(Ctx& cx, const Op& op, double emin, double emax, const s
   out->stats["tile_lpr"] = cheb_tile.lpr;
   out->stats["tile_rt"] = cheb_tile.rt;
   out->stats["tile_lds"] = cheb_tile.lds;
+  out->stats["tile_split"] = cheb_tile.split;
+  out->stats["filter_split_steps"] = (double)(cx.split_steps - split0);
   out->stats["smin"] = smin;
   out->stats["smax"] = smax;
   // the matrix is read once per launch (once per panel of a wide block)
@@ -2738,6 +2917,7 @@ static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
     r->stats["tile_lpr"] = wt.lpr;
     r->stats["tile_rt"] = wt.rt;
     r->stats["tile_lds"] = wt.lds;
+    r->stats["tile_split"] = wt.split;
   }
   return out;
 }

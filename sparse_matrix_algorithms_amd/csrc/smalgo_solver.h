@@ -86,6 +86,13 @@ struct Ctx {
   int last_launches_per_step = 1;  // kernel launches per filter step (panels)
   std::map<long long, float> tuned_ms;  // per-step time of tuned per-column tiles
   Staging staging;                      // host-callback operator transfers
+  // second stream for split filter steps (FilterTile::split = 2), created on
+  // first use; fork_ev / join_ev order it after / before the main stream
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  long long split_steps = 0;  // filter steps issued as two launches (statistics)
+  bool split_active = false;  // inside a fork/join pair of the two streams
+  int split_h = -1;           // column split of the last launch in it (0: one launch)
 };
 
 struct Mat {

@@ -134,3 +134,38 @@ def test_filter_every_tile(ctx, prec, monkeypatch):
                 Y = sm.apply_poly_block(dm, Q, co, smin, smax, precision=prec, ctx=ctx)
                 err = np.linalg.norm(Y - Yo) / np.linalg.norm(Yo)
                 assert err <= tol, (A.shape, b, v, l, rt, lds, err)
+
+
+def _split_tiles():
+    for v in (1, 2, 4, 8):
+        for l in (4, 8, 16, 32, 64):
+            for rt in (1, 2):
+                for lds in (0, 1):
+                    yield v, l, rt, lds
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_filter_split_tiles(ctx, prec, monkeypatch):
+    """Split steps (two launches over the column halves, one on each of the
+    context's streams), forced for every half-width tile the autotuner may
+    pick, against the FP64 oracle: widths 64 and 96 (FP32 splits 96 into 32 +
+    64 columns at a 128-byte line, FP64 into 48 + 48) and 40, whose rows do
+    not start on 128-byte lines, so the step runs as one launch."""
+    import scipy.sparse as sps
+
+    import sparse_matrix_algorithms_amd as sm
+    R = sps.random(300, 300, density=0.04, random_state=np.random.default_rng(7), format="csr")
+    mats = [xxz_sz0(10, 5.0, 4).tocsr(), (R + R.T + sps.identity(300)).tocsr()]
+    tol = 1e-12 if prec == "fp64" else 2e-4
+    for A in mats:
+        smin, smax = P.gershgorin_bounds(A)
+        dm = sm.DeviceMatrix(A, ctx)
+        co = P.build_coeffs(20, smin, smax, 0.5 * (smin + smax), smin * 0.3, smax * 0.3, "step")
+        for b in (64, 96, 40):
+            Q = gaussian_block(9, A.shape[0], b)
+            Yo = P.apply_poly_block(A, Q, co, smin, smax)
+            for v, l, rt, lds in _split_tiles():
+                monkeypatch.setenv("SMALGO_FILTER_TILE", f"{v},{l},{rt},{lds},2")
+                Y = sm.apply_poly_block(dm, Q, co, smin, smax, precision=prec, ctx=ctx)
+                err = np.linalg.norm(Y - Yo) / np.linalg.norm(Yo)
+                assert err <= tol, (A.shape, b, v, l, rt, lds, err)

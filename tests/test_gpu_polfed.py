@@ -46,6 +46,44 @@ def test_polfed_anderson_small(ctx, precision):
         assert np.max(np.abs(np.sort(lam) - np.sort(lo))) <= 1e-10 * anorm
 
 
+def test_polfed_split_steps(ctx, monkeypatch):
+    """The basis filter with split steps forced (each step as two 32-column
+    launches, one on each of the context's streams).  FP64: the same
+    eigenpairs as dense eigh and as the oracle.  FP32 (mixed path): the same
+    bits as the one-launch tile of the same shape, which runs the same
+    arithmetic per column."""
+    import sparse_matrix_algorithms_amd as sm
+    A = anderson3d(10, 16.5, 23)
+    n = A.shape[0]
+    ev = np.linalg.eigh(A.toarray())[0]
+    nev = 60
+    emin, emax = _window(ev, nev)
+    exact = ev[(ev >= emin) & (ev <= emax)]
+    smin, smax = P.gershgorin_bounds(A)
+    anorm = max(abs(smin), abs(smax))
+    Q1 = gaussian_block(2, n, 64)
+    kw = dict(s=64, smin=smin, smax=smax, emin=emin, emax=emax, mmax=5)
+    monkeypatch.setenv("SMALGO_FILTER_TILE", "4,8,1,1,2")
+    lam, X, res, st = sm.polfed(A, nev, 1000, Q1=Q1, precision="fp64", return_stats=True,
+                                ctx=ctx, **kw)
+    assert st["tile_split"] == 2 and st["filter_split_steps"] >= 1000, st
+    assert len(lam) == len(exact), st["line"]
+    assert np.max(np.abs(np.sort(lam) - exact)) <= 1e-10 * anorm
+    assert res.max() <= 1e-10 * anorm, st
+    R = A @ X - X * lam[None, :]
+    assert np.max(np.linalg.norm(R, axis=0)) <= 2e-10 * anorm
+    lo, Xo, ro = P.polfed(A, nev, 1000, Q0=Q1, **kw)
+    assert np.max(np.abs(np.sort(lam) - np.sort(lo))) <= 1e-10 * anorm
+    runs = []
+    for tile in ("4,8,1,1,2", "4,8,1,1"):
+        monkeypatch.setenv("SMALGO_FILTER_TILE", tile)
+        runs.append(sm.polfed(A, nev, 1000, Q1=Q1, precision="fp32", return_stats=True, ctx=ctx,
+                              **kw))
+    assert runs[0][3]["filter_split_steps"] >= 1000 and runs[1][3]["filter_split_steps"] == 0
+    for a, b in zip(runs[0][:3], runs[1][:3]):
+        assert np.array_equal(a, b)
+
+
 def test_polfed_lap1d_reference_style(ctx):
     """Lap1D count window as in polfed.jl:1223-1233, delta window at the centre."""
     import sparse_matrix_algorithms_amd as sm

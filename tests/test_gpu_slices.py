@@ -45,11 +45,13 @@ def test_slices_filter_fp64_vs_oracle(ctx, pw):
             assert err <= 1e-12, (r, i, err)
 
 
-@pytest.mark.parametrize("tile", ["4,4,1,1", "2,4,2,2", "1,8,4,0", "4,4,1,2", "8,4,1,1"])
+@pytest.mark.parametrize("tile", ["4,4,1,1", "2,4,2,2", "1,8,4,0", "4,4,1,2", "8,4,1,1",
+                                  "4,4,1,1,2", "2,8,1,0,2", "1,16,2,1,2"])
 def test_slices_filter_forced_tiles_vs_oracle(ctx, tile, monkeypatch):
     """Packed panels (dense n x 32 slabs) with forced tiles: column chunks of
     8 to 32 columns, CSR per lane, staged in LDS, and staged with own-row
-    prefetch."""
+    prefetch; and split steps (each panel step as two 16-column launches on
+    the context's two streams, captured into the HIP graphs as two branches)."""
     import sparse_matrix_algorithms_amd as sm
     monkeypatch.setenv("SMALGO_SLICE_TILE", tile)
     A = anderson3d(8, 16.5, 5)
@@ -65,8 +67,14 @@ def test_slices_filter_forced_tiles_vs_oracle(ctx, tile, monkeypatch):
         assert err <= 1e-12, (tile, i, err)
 
 
-def test_slices_filter_graph_equals_eager(ctx):
+@pytest.mark.parametrize("tile", [None, "4,4,1,1,2"])
+def test_slices_filter_graph_equals_eager(ctx, tile, monkeypatch):
+    """Graph replays give the same bits as eager launches, also with split
+    steps (two streams: the graph holds both branches, forked and joined by
+    events)."""
     import sparse_matrix_algorithms_amd as sm
+    if tile:
+        monkeypatch.setenv("SMALGO_SLICE_TILE", tile)
     A = anderson3d(8, 16.5, 4)
     n = A.shape[0]
     smin, smax = P.gershgorin_bounds(A)
