@@ -233,22 +233,16 @@ def main():
                   kpm_seconds=round(t_k, 3), upload_seconds=round(t_up, 3),
                   headline=dict(precision=head_prec, **head))
     # dominant kernel: the fused Clenshaw step of the basis build
-    launches = st.get("filter_launches", st["filter_steps"])
-    avg_launch = st["t_filter"] / max(launches, 1)
-    bytes_per_launch = st["spmm_bytes"] / max(launches, 1)
-    achieved = bytes_per_launch / avg_launch / 1e9
+    steps = st.get("filter_launches", st["filter_steps"])
     tname = "float" if head_prec == "fp32" else "double"
     kname = kernel_name(st, tname)
-    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs of
+    # HBM bytes per step from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs of
     # the same kernel and tile on this workload (scaled by the copy-kernel calibration)
     traffic = None
     tfile = ROOT / "profiles" / "clenshaw_anderson_L48_b64_traffic.json"
     if tfile.exists() and args.L == 48 and args.s == 64:
         traffic = json.loads(tfile.read_text()).get(kname, {}).get("traffic_bytes")
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    kernel=kname, bytes_per_launch=bytes_per_launch,
-                    avg_launch_us=round(avg_launch * 1e6, 3), launches=int(launches))
+    roofline = roofline_dict(st, head_prec, False, steps, traffic=traffic)
 
     def same(a, b):
         return bool(len(a) == len(b) and (len(a) == 0 or
@@ -319,13 +313,15 @@ def kernel_name(st, tname, percol=False):
     """Template name of the fused filter kernel the autotuner picked: the CSR
     tile read per lane (clenshaw_kernel) or staged in LDS (clenshaw_lds_kernel,
     whose last flag is the prefetch form: own-row loads before the staging
-    barrier, used by tiles of one or two rows per team)."""
+    barrier, used by tiles of one or two rows per team); " x2" when each step
+    runs as two concurrent launches over the column halves."""
     lds, rt = int(st.get("tile_lds", 0)), int(st["tile_rt"])
     args = (f"{tname},{int(st['tile_vec'])},{int(st['tile_lpr'])},{rt},"
             f"{'true' if percol else 'false'}")
+    x2 = " x2" if int(st.get("tile_split", 1)) == 2 else ""
     if lds == 0:
-        return f"clenshaw_kernel<{args}>"
-    return f"clenshaw_lds_kernel<{args},{'true' if lds == 2 and rt <= 2 else 'false'}>"
+        return f"clenshaw_kernel<{args}>{x2}"
+    return f"clenshaw_lds_kernel<{args},{'true' if lds == 2 and rt <= 2 else 'false'}>{x2}"
 
 
 def cpu_info():
@@ -433,20 +429,30 @@ def cpu_baseline(A, b, smin, smax, emin, emax, steps, pairs, seconds, what, bloc
                         f"; projected CPU solve {t_solve:.1f}s {what}"))
 
 
-def filter_roofline(st, prec, percol, bytes_key="spmm_bytes", launches=None):
-    """Achieved algorithmic bandwidth of the fused Clenshaw kernel from the
-    solver's event-timed filter phase and its modelled bytes."""
-    if launches is None:
-        launches = st.get("filter_launches", st["filter_steps"])
-    avg_launch = st["t_filter"] / max(launches, 1)
-    bpl = st[bytes_key] / max(launches, 1)
-    achieved = bpl / avg_launch / 1e9
+def roofline_dict(st, prec, percol, steps, bytes_key="spmm_bytes", traffic=None):
+    """Achieved algorithmic bandwidth of the fused Clenshaw step from the
+    solver's event-timed filter phase and its modelled bytes.  A step is one
+    launch, or with a split tile two concurrent launches over the column
+    halves (launches_per_step = 2; the matrix counts once in its algorithmic
+    bytes, and the step time runs from the first launch's start to the last
+    one's end)."""
+    avg_step = st["t_filter"] / max(steps, 1)
+    bps = st[bytes_key] / max(steps, 1)
+    achieved = bps / avg_step / 1e9
     tname = "float" if prec == "fp32" else "double"
     kname = kernel_name(st, tname, percol)
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=kname,
-                bytes_per_launch=bpl, avg_launch_us=round(avg_launch * 1e6, 3),
-                launches=int(launches))
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=kname,
+                launches_per_step=2 if int(st.get("tile_split", 1)) == 2 else 1,
+                bytes_per_step=bps, avg_step_us=round(avg_step * 1e6, 3), steps=int(steps))
+
+
+def filter_roofline(st, prec, percol, bytes_key="spmm_bytes", launches=None):
+    """roofline_dict over the filter's sequential launches (panels of a wide
+    block run one after another; each is one step)."""
+    if launches is None:
+        launches = st.get("filter_launches", st["filter_steps"])
+    return roofline_dict(st, prec, percol, launches, bytes_key)
 
 
 def _ctx_matrix(A):

@@ -8,6 +8,13 @@ so that FETCH_SIZE / WRITE_SIZE counters can be scaled to bytes.
     python -m sparse_matrix_algorithms_amd.profile_filter --L 48 --b 64 --prec fp32 --degree 50
     python -m sparse_matrix_algorithms_amd.profile_filter --lattice xxz --L 20 --seed 20 --b 128
     python -m sparse_matrix_algorithms_amd.profile_filter --copy-sweep
+    python -m sparse_matrix_algorithms_amd.profile_filter --trace-span run_kernel_trace.csv
+
+``--trace-span`` reads a rocprofv3 kernel trace (CSV) instead and prints the
+time per step of the most frequent filter kernel: the union of its dispatch
+intervals divided by the number of steps, where a step of a split tile is two
+concurrent dispatches (``--launches-per-step 2``, the default) whose
+individual durations overlap.
 
 With ``--slices N`` it runs the packed multi-slice filter instead: N slices of
 widths 8, 64, 16, 32, ... (the batched-slices benchmark's widths, cycled),
@@ -44,7 +51,12 @@ def main():
                     help="xxz: random-field XXZ chain, Sz=0 sector, length --L (h=5, seed --seed)")
     ap.add_argument("--copy-sweep", action="store_true",
                     help="only time the copy kernel over buffers of 16 MiB to 4 GiB")
+    ap.add_argument("--trace-span", default=None,
+                    help="rocprofv3 kernel-trace CSV: per-step time of the filter kernel")
+    ap.add_argument("--launches-per-step", type=int, default=2)
     args = ap.parse_args()
+    if args.trace_span:
+        return trace_span(args.trace_span, args.launches_per_step)
 
     import sparse_matrix_algorithms_amd as sm
     from sparse_matrix_algorithms_amd.lattices import anderson3d, gaussian_block, xxz_sz0
@@ -94,6 +106,33 @@ def main():
     ctx.check(ctx.lib.smalgo_copy_bench(ctx.h, nbytes, 20, C.byref(sec)))
     print(f"copy kernel: {nbytes} B read + {nbytes} B written per launch, {sec.value * 1e6:.1f} us "
           f"-> {2 * nbytes / sec.value / 1e9:.1f} GB/s", flush=True)
+
+
+def trace_span(path, per_step):
+    """Per-step time of the most frequent Clenshaw kernel in a kernel trace."""
+    import collections
+    import csv
+    import json
+    rows = list(csv.DictReader(open(path)))
+    cnt = collections.Counter(r["Kernel_Name"] for r in rows if "clenshaw" in r["Kernel_Name"])
+    name = cnt.most_common(1)[0][0]
+    iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Stream_Id"])
+                for r in rows if r["Kernel_Name"] == name)
+    union, (cs, ce) = 0, iv[0][:2]
+    for s0, e0, _ in iv[1:]:
+        if s0 > ce:
+            union += ce - cs
+            cs, ce = s0, e0
+        else:
+            ce = max(ce, e0)
+    union += ce - cs
+    steps = len(iv) // per_step
+    res = dict(kernel=name.split("(")[0], dispatches=len(iv), launches_per_step=per_step,
+               steps=steps, avg_dispatch_us=sum(e - s0 for s0, e, _ in iv) / len(iv) / 1e3,
+               union_ms=union / 1e6, avg_step_us=union / max(steps, 1) / 1e3,
+               streams=dict(collections.Counter(x[2] for x in iv)))
+    print(json.dumps(res), flush=True)
+    return 0
 
 
 if __name__ == "__main__":
