@@ -554,12 +554,14 @@ def bench_slices(args):
     per = count_valid(slices, outs)
     total = sum(per)
     log(f"[bench] batch of {ns}: {dt:.2f}s, {total} valid pairs, t_filter={st['t_filter']:.2f}s "
-        f"panel width {st['slice_panel_width']:.0f}")
+        f"panel width {st['slice_panel_width']:.0f}, paired chains in {st.get('slice_paired', 0):.0f} "
+        f"block steps")
     detail = dict(n=n, nnz=dm.nnz, upload_seconds=round(t_up, 3), precision=prec,
                   graphs=not args.eager, t_filter=st["t_filter"], t_orth=st["t_orth"],
                   t_extract=st["t_extract"], t_refine=st["t_refine"],
                   panel_width=st["slice_panel_width"], replans=st["slice_replans"],
-                  graphs_per_block=st["slice_graphs"], retry=retry,
+                  graphs_per_block=st["slice_graphs"], paired_steps=st.get("slice_paired", 0),
+                  retry=retry,
                   filter_gbs=round(st["slice_filter_bytes"] / st["t_filter"] / 1e9, 1),
                   slices=[dict(centre=d["sigma"], interval=[d["emin"], d["emax"]], nev=d["nev"],
                                s=d["s"], degree=d["degree"], mmax=d["mmax"], valid=v,

@@ -1534,7 +1534,15 @@ struct SliceFilter {
   SlicePack pk;
   std::vector<int> widths;
   DBuf<T> Qp, Yp, P0, P1, tab;  // panels, ping-pong buffers, coefficient table
+  DBuf<T> P0s, P1s;             // ping-pong buffers of the side chain (paired panels)
   std::vector<std::vector<Step>> steps;
+  // Paired chains: panels whose steps do not split and whose width lets two
+  // of them (Q, B1, B2 each, plus the matrix) fit the cache budget run as two
+  // independent chains, one per stream (chain[1] on the side stream with
+  // P0s/P1s), their launches interleaved.  The other panels (solo) run first,
+  // one after another, splitting their own steps where their tile does.
+  bool paired = false;
+  std::vector<int> solo, chain[2];
   std::vector<hipGraphExec_t> execs;
   std::vector<hipGraph_t> graphs;
   int graph_nodes_max = 2048;  // launches per captured graph
@@ -1585,7 +1593,63 @@ struct SliceFilter {
     for (const auto& P : pk.pan) fill_block<T>(cx.stream, n, P.w, Qp.p + P.base, P.w, T(0));
     fill_block<T>(cx.stream, n, wmax, P0.p, wmax, T(0));
     fill_block<T>(cx.stream, n, wmax, P1.p, wmax, T(0));
+    plan_chains();
+    if (paired) {
+      P0s.ensure((size_t)n * wmax);
+      P1s.ensure((size_t)n * wmax);
+      fill_block<T>(cx.stream, n, wmax, P0s.p, wmax, T(0));
+      fill_block<T>(cx.stream, n, wmax, P1s.p, wmax, T(0));
+    }
     SMALGO_HIP(hipStreamSynchronize(cx.stream));
+  }
+
+  // Panels go to the chain with fewer steps so far, longest first.
+  // SMALGO_SLICE_PAIR=0 keeps every panel solo.
+  void plan_chains() {
+    solo.clear();
+    chain[0].clear();
+    chain[1].clear();
+    const char* e = std::getenv("SMALGO_SLICE_PAIR");
+    const double mat = (double)op.A->nnz * (sizeof(T) + 4);
+    std::vector<int> pair;
+    for (size_t q = 0; q < pk.pan.size(); ++q) {
+      const int w = pk.pan[q].w;
+      const bool fits = 2.0 * 3.0 * n * w * sizeof(T) + mat <= 240e6;
+      const bool splits = pk.pan[q].tile.split == 2 && split_ok(w, sizeof(T), {P0.p, P1.p}, {w});
+      if (!splits && fits && !(e && e[0] == '0'))
+        pair.push_back((int)q);
+      else
+        solo.push_back((int)q);
+    }
+    paired = pair.size() >= 2;
+    if (!paired) {
+      solo.insert(solo.end(), pair.begin(), pair.end());
+      std::sort(solo.begin(), solo.end());
+      return;
+    }
+    std::stable_sort(pair.begin(), pair.end(),
+                     [&](int a, int b) { return steps[a].size() > steps[b].size(); });
+    size_t load[2] = {0, 0};
+    for (int q : pair) {
+      const int c = load[1] < load[0] ? 1 : 0;
+      chain[c].push_back(q);
+      load[c] += steps[q].size();
+    }
+  }
+
+  // every launch of one run in issue order: (panel, step, side chain)
+  template <typename F>
+  void each_launch(F&& f) {
+    for (int q : solo)
+      for (const Step& st : steps[q]) f(q, st, false);
+    std::vector<std::pair<int, size_t>> seq[2];
+    for (int c = 0; c < 2; ++c)
+      for (int q : chain[c])
+        for (size_t k = 0; k < steps[q].size(); ++k) seq[c].push_back({q, k});
+    const size_t m = std::max(seq[0].size(), seq[1].size());
+    for (size_t i = 0; i < m; ++i)
+      for (int c = 0; c < 2; ++c)
+        if (i < seq[c].size()) f(seq[c][i].first, steps[seq[c][i].first][seq[c][i].second], c == 1);
   }
 
   // Autotuner for the per-column kernel at width w: time a few steps of
@@ -1706,7 +1770,7 @@ struct SliceFilter {
       }
   }
 
-  void launch(int p, const Step& st) {
+  void launch(int p, const Step& st, bool side = false) {
     const SlicePack::Panel& P = pk.pan[p];
     const int pw = P.w;
     T* Qn = Qp.p + P.base;
@@ -1714,12 +1778,18 @@ struct SliceFilter {
     auto ptr = [&](int role) -> T* {
       switch (role) {
         case 1: return Qn;
-        case 2: return P0.p;
-        case 3: return P1.p;
+        case 2: return side ? P0s.p : P0.p;
+        case 3: return side ? P1s.p : P1.p;
         case 4: return Yn;
         default: return nullptr;
       }
     };
+    if (side) {  // paired chains: the side chain's panel, one launch on the side stream
+      launch_clenshaw_step_cols<T>(cx.side, *op.A, pw, Qn, pw, tab.p + st.cq, ptr(st.x1), pw,
+                                   (T)st.a1, (T)st.b1, ptr(st.x2), pw, (T)st.s2, ptr(st.out),
+                                   pw, P.tile);
+      return;
+    }
     launch_step_cols<T>(cx, *op.A, pw, Qn, pw, tab.p + st.cq, ptr(st.x1), pw, (T)st.a1,
                         (T)st.b1, ptr(st.x2), pw, (T)st.s2, ptr(st.out), pw, P.tile);
   }
@@ -1735,9 +1805,8 @@ struct SliceFilter {
   // graph_nodes_max kernels (kernel nodes only) and replayed afterwards.
   void run(bool graph) {
     if (!graph) {
-      SplitScope split(cx, any_split());
-      for (size_t p = 0; p < pk.pan.size(); ++p)
-        for (const Step& st : steps[p]) launch((int)p, st);
+      SplitScope split(cx, any_split() || paired);
+      each_launch([&](int p, const Step& st, bool side) { launch(p, st, side); });
       return;
     }
     if (execs.empty()) capture();
@@ -1749,19 +1818,25 @@ struct SliceFilter {
   }
 
   void capture() {
-    std::vector<std::pair<int, size_t>> all;
-    for (size_t p = 0; p < pk.pan.size(); ++p)
-      for (size_t k = 0; k < steps[p].size(); ++k) all.push_back({(int)p, k});
+    struct L {
+      int p;
+      const Step* st;
+      bool side;
+    };
+    std::vector<L> all;
+    each_launch([&](int p, const Step& st, bool side) { all.push_back({p, &st, side}); });
+    const bool two = any_split() || paired;
     for (size_t b = 0; b < all.size(); b += graph_nodes_max) {
       const size_t e = std::min(all.size(), b + (size_t)graph_nodes_max);
       hipGraph_t g = nullptr;
-      if (any_split()) side_stream(cx);
+      if (two) side_stream(cx);
       SMALGO_HIP(hipStreamBeginCapture(cx.stream, hipStreamCaptureModeThreadLocal));
       try {
-        // split panels: the side stream joins the capture through the fork
-        // event and leaves it through the join event (graph edges)
-        SplitScope split(cx, any_split());
-        for (size_t k = b; k < e; ++k) launch(all[k].first, steps[all[k].first][all[k].second]);
+        // split panels and paired chains: the side stream joins the capture
+        // through the fork event and leaves it through the join event (graph
+        // edges), so each graph holds both streams' launches as two branches
+        SplitScope split(cx, two);
+        for (size_t k = b; k < e; ++k) launch(all[k].p, *all[k].st, all[k].side);
       } catch (...) {
         (void)hipStreamEndCapture(cx.stream, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -2820,7 +2895,7 @@ static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
   std::vector<int> planned;  // active slices the current plan was built for
   std::vector<long long> fsteps(ns, 0);
   double gemm_flops = 0, filter_bytes = 0, filter_launches = 0;
-  int replans = 0, max_launch_graphs = 0;
+  int replans = 0, max_launch_graphs = 0, paired_runs = 0;
   for (int j = 0;; ++j) {
     std::vector<int> act;
     for (int i = 0; i < ns; ++i)
@@ -2844,6 +2919,7 @@ static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
     }
     sf.run(graph);
     tm.end(hf);
+    paired_runs += sf.paired ? 1 : 0;
     max_launch_graphs = std::max(max_launch_graphs, (int)sf.execs.size());
     filter_bytes += sf.bytes();
     filter_launches += (double)sf.launches();
@@ -2918,6 +2994,7 @@ static std::vector<Result*> polfed_slices_run(Ctx& cx, const Op& op,
     r->stats["tile_rt"] = wt.rt;
     r->stats["tile_lds"] = wt.lds;
     r->stats["tile_split"] = wt.split;
+    r->stats["slice_paired"] = paired_runs;  // block steps whose panels ran as two chains
   }
   return out;
 }

@@ -270,7 +270,7 @@ _STATS = ("t_total", "t_filter", "t_orth", "t_extract", "t_refine", "filter_step
           "filter_launches",
           "block_steps", "basis_dim", "iterations", "spmm_bytes", "gemm_flops",
           "refine_filter_steps", "locked", "smin", "smax", "warn_interval", "tile_vec",
-          "tile_lpr", "tile_rt", "tile_lds", "tile_split", "filter_split_steps",
+          "tile_lpr", "tile_rt", "tile_lds", "tile_split", "filter_split_steps", "slice_paired",
           "slice_panel_width", "slice_replans", "slice_graphs",
           "slice_filter_bytes", "slice_gemm_flops", "slice_launches", "refine_iterations",
           "t_refine_host")

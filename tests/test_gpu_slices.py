@@ -167,6 +167,65 @@ def test_polfed_slices_eager_matches_graph(ctx):
         assert np.array_equal(rg, re_)
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_polfed_slices_paired_chains(ctx, use_graph, monkeypatch):
+    """Panels run as two concurrent chains (one per stream, launches
+    interleaved) give the same bits as one chain, graph-captured and eager,
+    and the eigenpairs of dense eigh.  8-column panels: too narrow to split
+    their own steps, so the panels pair."""
+    import sparse_matrix_algorithms_amd as sm
+    monkeypatch.setenv("SMALGO_SLICE_PW", "8")
+    A = anderson3d(10, 16.5, 22)
+    ev = np.linalg.eigh(A.toarray())[0]
+    smin, smax = P.gershgorin_bounds(A)
+    anorm = max(abs(smin), abs(smax))
+    specs = [(-3.0, 12, 4, 1200, 3), (0.0, 30, 8, 500, 3), (2.5, 32, 16, 800, 3.5)]
+    slices = []
+    for c, nev, s, K, f in specs:
+        a, b = _count_window(ev, c, nev)
+        slices.append(dict(emin=a, emax=b, nev=nev, s=s, degree=K,
+                           mmax=math.ceil(f * nev / s)))
+    runs = {}
+    for pair in ("1", "0"):
+        monkeypatch.setenv("SMALGO_SLICE_PAIR", pair)
+        runs[pair] = sm.polfed_slices(A, slices, smin=smin, smax=smax, precision="fp64", seed=3,
+                                      use_graph=use_graph, ctx=ctx)
+    assert runs["1"][0][3]["slice_paired"] >= 1 and runs["0"][0][3]["slice_paired"] == 0
+    for i, d in enumerate(slices):
+        lam, X, res, st = runs["1"][i]
+        exact = ev[(ev >= d["emin"]) & (ev <= d["emax"])]
+        assert len(lam) == d["nev"], (i, st["line"])
+        assert np.max(np.abs(np.sort(lam) - exact)) <= 1e-10 * anorm, i
+        assert res.max() <= 1e-10 * anorm, (i, st)
+        for a, b in zip(runs["1"][i][:3], runs["0"][i][:3]):
+            assert np.array_equal(a, b), i
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_polfed_slices_pairing_bits(ctx, precision, monkeypatch):
+    """Slices of 4 to 32 columns packed freely (solo panels that split their
+    own steps and narrow panels paired into two chains, as the packer and the
+    autotuner decide): the same bits with and without paired chains."""
+    import sparse_matrix_algorithms_amd as sm
+    A = anderson3d(10, 16.5, 22)
+    ev = np.linalg.eigh(A.toarray())[0]
+    smin, smax = P.gershgorin_bounds(A)
+    slices = []
+    for c, nev, s, K in [(-3.0, 12, 4, 900), (0.0, 40, 32, 500), (2.5, 24, 8, 700),
+                         (-1.5, 20, 16, 600), (1.2, 16, 4, 800)]:
+        a, b = _count_window(ev, c, nev)
+        slices.append(dict(emin=a, emax=b, nev=nev, s=s, degree=K, mmax=math.ceil(3 * nev / s)))
+    runs = {}
+    for pair in ("1", "0"):
+        monkeypatch.setenv("SMALGO_SLICE_PAIR", pair)
+        runs[pair] = sm.polfed_slices(A, slices, smin=smin, smax=smax, precision=precision,
+                                      seed=5, ctx=ctx)
+    assert runs["0"][0][3]["slice_paired"] == 0
+    for i in range(len(slices)):
+        for a, b in zip(runs["1"][i][:3], runs["0"][i][:3]):
+            assert np.array_equal(a, b), i
+
+
 def test_polfed_slices_bad_args(ctx):
     import sparse_matrix_algorithms_amd as sm
     A = anderson3d(6, 16.5, 1)
