@@ -521,22 +521,31 @@ static std::vector<FilterTile> tile_candidates(int b, size_t elem, bool prefetch
     }
   };
   add(b, 1);
-  const int h = split ? split_half(b, elem) : 0;
+  static const bool no_split = [] {
+    const char* e = std::getenv("SMALGO_NO_SPLIT");
+    return e && e[0] == '1';
+  }();
+  const int h = split && !no_split ? split_half(b, elem) : 0;
   if (h > 0) add(h, 2);
   return cand;
 }
 
 // Times step(tile) for every candidate in `sweeps` sweeps of `reps` steps, each
 // after one warm step; a tile's time (ms per step, into tmin) is its fastest
-// sweep, so clock ramps and one-off stalls do not pick the tile.  Returns the
-// index of the fastest.  SMALGO_TUNE_LOG=1 prints every time and the pick.
+// sweep, so clock ramps and one-off stalls do not pick the tile.  Sweeps after
+// the first re-time only the candidates within 25% of the best time so far.
+// Returns the index of the fastest.  SMALGO_TUNE_LOG=1 prints every time and
+// the pick.
 template <typename StepFn>
 static size_t time_tiles(Ctx& cx, const std::vector<FilterTile>& cand, int sweeps, int reps,
                          StepFn&& step, std::vector<float>& tmin, const std::string& what) {
   Event e0, e1, e1s;
   tmin.assign(cand.size(), 1e30f);
-  for (int sweep = 0; sweep < sweeps; ++sweep)
+  for (int sweep = 0; sweep < sweeps; ++sweep) {
+    // later sweeps re-time only the tiles within 25% of the best so far
+    const float cut = 1.25f * *std::min_element(tmin.begin(), tmin.end());
     for (size_t i = 0; i < cand.size(); ++i) {
+      if (sweep > 0 && tmin[i] > cut) continue;
       // A split tile's steps run on both streams inside one fork/join pair,
       // as in the recurrence.  Their time runs from e0 on the main stream to
       // the later of the two streams' end events, so the fork and the join
@@ -563,6 +572,7 @@ static size_t time_tiles(Ctx& cx, const std::vector<FilterTile>& cand, int sweep
       }
       tmin[i] = std::min(tmin[i], ms / (float)reps);
     }
+  }
   size_t best = 0;
   for (size_t i = 1; i < cand.size(); ++i)
     if (tmin[i] < tmin[best]) best = i;
