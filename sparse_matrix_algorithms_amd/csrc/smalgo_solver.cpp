@@ -533,7 +533,8 @@ static std::vector<FilterTile> tile_candidates(int b, size_t elem, bool prefetch
 // Times step(tile) for every candidate in `sweeps` sweeps of `reps` steps, each
 // after one warm step; a tile's time (ms per step, into tmin) is its fastest
 // sweep, so clock ramps and one-off stalls do not pick the tile.  Sweeps after
-// the first re-time only the candidates within 25% of the best time so far.
+// the first re-time only the candidates within 25% of the best time so far,
+// over twice the steps.
 // Returns the index of the fastest.  SMALGO_TUNE_LOG=1 prints every time and
 // the pick.
 template <typename StepFn>
@@ -542,8 +543,10 @@ static size_t time_tiles(Ctx& cx, const std::vector<FilterTile>& cand, int sweep
   Event e0, e1, e1s;
   tmin.assign(cand.size(), 1e30f);
   for (int sweep = 0; sweep < sweeps; ++sweep) {
-    // later sweeps re-time only the tiles within 25% of the best so far
+    // later sweeps re-time only the tiles within 25% of the best so far, with
+    // twice the steps
     const float cut = 1.25f * *std::min_element(tmin.begin(), tmin.end());
+    const int nrep = sweep > 0 ? 2 * reps : reps;
     for (size_t i = 0; i < cand.size(); ++i) {
       if (sweep > 0 && tmin[i] > cut) continue;
       // A split tile's steps run on both streams inside one fork/join pair,
@@ -556,7 +559,7 @@ static size_t time_tiles(Ctx& cx, const std::vector<FilterTile>& cand, int sweep
         SplitScope scope(cx, sp);
         step(cand[i]);  // warm
         SMALGO_HIP(hipEventRecord(e0, cx.stream));
-        for (int r = 0; r < reps; ++r) step(cand[i]);
+        for (int r = 0; r < nrep; ++r) step(cand[i]);
         SMALGO_HIP(hipEventRecord(e1, cx.stream));
         two = sp && cx.split_active;
         if (two) SMALGO_HIP(hipEventRecord(e1s, cx.side));
@@ -570,7 +573,7 @@ static size_t time_tiles(Ctx& cx, const std::vector<FilterTile>& cand, int sweep
         SMALGO_HIP(hipEventElapsedTime(&ms2, e0, e1s));
         ms = std::max(ms, ms2);
       }
-      tmin[i] = std::min(tmin[i], ms / (float)reps);
+      tmin[i] = std::min(tmin[i], ms / (float)nrep);
     }
   }
   size_t best = 0;
