@@ -1591,6 +1591,15 @@ struct SliceFilter {
       return (double)ms;
     };
     pk = pack_slices(w, deg, wmax, wfix, t);
+    // Narrower panels that pair into two chains (below) may beat the
+    // cache-width plan: plan once more at the widest pairable width with the
+    // step times scaled by the measured pairing gain, and keep the cheaper.
+    const int wp = pair_width(wmax);
+    if (wfix <= 0 && wp >= 8 && wp < wmax) {
+      const double r = pair_ratio(wp);
+      SlicePack alt = pack_slices(w, deg, wp, 0, [&](int width) { return r * t(width); });
+      if (alt.pred_ms < pk.pred_ms) pk = std::move(alt);
+    }
     for (auto& P : pk.pan) {
       float ms = 0;
       P.tile = tune(P.w, &ms);
@@ -1622,14 +1631,14 @@ struct SliceFilter {
     solo.clear();
     chain[0].clear();
     chain[1].clear();
-    const char* e = std::getenv("SMALGO_SLICE_PAIR");
     const double mat = (double)op.A->nnz * (sizeof(T) + 4);
+    const bool on = pairing_enabled();
     std::vector<int> pair;
     for (size_t q = 0; q < pk.pan.size(); ++q) {
       const int w = pk.pan[q].w;
       const bool fits = 2.0 * 3.0 * n * w * sizeof(T) + mat <= 240e6;
       const bool splits = pk.pan[q].tile.split == 2 && split_ok(w, sizeof(T), {P0.p, P1.p}, {w});
-      if (!splits && fits && !(e && e[0] == '0'))
+      if (!splits && fits && on)
         pair.push_back((int)q);
       else
         solo.push_back((int)q);
@@ -1663,6 +1672,67 @@ struct SliceFilter {
     for (size_t i = 0; i < m; ++i)
       for (int c = 0; c < 2; ++c)
         if (i < seq[c].size()) f(seq[c][i].first, steps[seq[c][i].first][seq[c][i].second], c == 1);
+  }
+
+  static bool pairing_enabled() {
+    const char* e = std::getenv("SMALGO_SLICE_PAIR");
+    return !(e && e[0] == '0');
+  }
+
+  // widest panel (multiple of 4, at most wmax) two of which (Q, B1, B2 each)
+  // fit the 240 MB cache budget with the matrix; 0 when pairing is off
+  int pair_width(int wmax) const {
+    if (!pairing_enabled()) return 0;
+    const double mat = (double)op.A->nnz * (sizeof(T) + 4);
+    const double per_col = 2.0 * 3.0 * n * sizeof(T);
+    const int w = (int)std::min<double>(wmax, std::floor((240e6 - mat) / per_col));
+    return w > 0 ? w / 4 * 4 : 0;
+  }
+
+  // Step time of two w-wide panels run as two concurrent chains (one per
+  // stream, each with the tuned one-launch tile of width w), over twice one
+  // chain's step time: the pairing gain the packer plans with (cached per
+  // precision, width and structure).
+  double pair_ratio(int w) {
+    const long long key = tune_key_mix({(long long)sizeof(T), (long long)w, op.mat_id, 2});
+    auto it = cx.tuned_ms.find(key);
+    if (it != cx.tuned_ms.end()) return it->second;
+    float t1 = 0.f;
+    FilterTile tl = tune(w, &t1);
+    double r = 1.0;
+    if (tl.split != 2 || !split_ok(w, sizeof(T), {P0.p, P1.p}, {w})) {
+      tl.split = 1;
+      DBuf<T> co((size_t)w), qa((size_t)n * w), qb((size_t)n * w), a0((size_t)n * w),
+          a1((size_t)n * w), b0((size_t)n * w), b1((size_t)n * w);
+      fill_block<T>(cx.stream, 1, w, co.p, w, T(0.25));
+      for (T* q : {qa.p, qb.p, a0.p, a1.p, b0.p, b1.p}) fill_block<T>(cx.stream, n, w, q, w, T(0));
+      const int reps = 8;
+      Event e0, e1, e1s;
+      float best = 1e30f;
+      for (int sweep = 0; sweep < 2; ++sweep) {
+        float ms = 0.f, ms2 = 0.f;
+        {
+          SplitScope scope(cx, true);
+          for (int r2 = 0; r2 <= reps; ++r2) {
+            if (r2 == 1) SMALGO_HIP(hipEventRecord(e0, cx.stream));  // after a warm step
+            launch_clenshaw_step_cols<T>(cx.stream, *op.A, w, qa.p, w, co.p, a0.p, w, T(0.1),
+                                         T(0.1), a1.p, w, T(1), a1.p, w, tl);
+            launch_clenshaw_step_cols<T>(cx.side, *op.A, w, qb.p, w, co.p, b0.p, w, T(0.1),
+                                         T(0.1), b1.p, w, T(1), b1.p, w, tl);
+          }
+          SMALGO_HIP(hipEventRecord(e1, cx.stream));
+          SMALGO_HIP(hipEventRecord(e1s, cx.side));
+        }
+        SMALGO_HIP(hipEventSynchronize(e1));
+        SMALGO_HIP(hipEventSynchronize(e1s));
+        SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
+        SMALGO_HIP(hipEventElapsedTime(&ms2, e0, e1s));
+        best = std::min(best, std::max(ms, ms2) / (float)reps);
+      }
+      r = std::min(1.0, (double)best / (2.0 * t1));
+    }
+    cx.tuned_ms[key] = (float)r;
+    return r;
   }
 
   // Autotuner for the per-column kernel at width w: time a few steps of
