@@ -283,13 +283,13 @@ template void apply_A<float>(Ctx&, const Op&, int, const float*, int, float*, in
 // launches over the column halves [0, h) and [h, b), the first on the
 // context's stream and the second on its side stream.  The halves share no
 // column, so their recurrences are independent; run concurrently, each chain
-// fills the other's launch ramp and drain.  h is b/2 rounded down to whole
-// 128-byte lines, and the split is made only when every block pointer and
+// fills the other's launch ramp and drain.  h is b/2 rounded to the nearest
+// whole 128-byte line, and the split is made only when every block pointer and
 // leading dimension keeps the halves on separate cache lines.
 // ---------------------------------------------------------------------------
 static int split_half(int b, size_t elem) {
   const int g = (int)(128 / elem);
-  const int h = (b / 2) / g * g;
+  const int h = (b + g) / (2 * g) * g;  // the whole-line boundary nearest b/2
   return (h >= g && b - h >= g) ? h : 0;
 }
 

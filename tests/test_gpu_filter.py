@@ -148,8 +148,8 @@ def _split_tiles():
 def test_filter_split_tiles(ctx, prec, monkeypatch):
     """Split steps (two launches over the column halves, one on each of the
     context's streams), forced for every half-width tile the autotuner may
-    pick, against the FP64 oracle: widths 64 and 96 (FP32 splits 96 into 32 +
-    64 columns at a 128-byte line, FP64 into 48 + 48) and 40, whose rows do
+    pick, against the FP64 oracle: widths 64 and 96 (FP32 splits 96 into 64 +
+    32 columns at a 128-byte line, FP64 into 48 + 48) and 40, whose rows do
     not start on 128-byte lines, so the step runs as one launch."""
     import scipy.sparse as sps
 
