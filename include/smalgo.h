@@ -54,6 +54,11 @@ typedef int (*smalgo_matblock_fn)(void* user, int64_t n, int64_t b, const double
                                   double* Y, int64_t ldy);
 
 /* ---- context ---------------------------------------------------------- */
+/* One GPU, one HIP stream (default priority) for all work of the context; a
+ * second compute stream is created on the first filter step the autotuner
+ * runs as two concurrent launches over the column halves, and every entry
+ * point joins it back into the first before it returns.  Calls on one
+ * context must not overlap; contexts are independent. */
 int smalgo_ctx_create(int device, smalgo_ctx** out);
 void smalgo_ctx_destroy(smalgo_ctx* ctx);
 const char* smalgo_last_error(const smalgo_ctx* ctx);
