@@ -2,7 +2,8 @@
 the GIL inside every library call): each thread's POLFED solves (FP64 and
 mixed) and its large Rayleigh–Ritz result downloads (>= 128 MiB, the pinned
 chunked path whose host buffers the threads share under a lock) give the same
-answers as the same calls made one after the other."""
+answers as the same calls made one after the other; also with split filter
+steps, each context then running two streams."""
 import math
 from concurrent.futures import ThreadPoolExecutor
 
@@ -15,7 +16,7 @@ from sparse_matrix_algorithms_amd.lattices import anderson3d, gaussian_block
 pytestmark = pytest.mark.gpu
 
 
-def _job(sm, ctx, A, Abig, Y):
+def _job(sm, ctx, A, Abig, Y, s=16, mmax=9):
     """Every call of one thread: returns the results to compare."""
     smin, smax = P.gershgorin_bounds(A)
     anorm = max(abs(smin), abs(smax))
@@ -27,8 +28,8 @@ def _job(sm, ctx, A, Abig, Y):
     dm = sm.DeviceMatrix(A, ctx)
     try:
         for prec in ("fp64", "fp32"):
-            lam, X, res = sm.polfed(dm, 36, math.ceil(8 / h), s=16, smin=smin, smax=smax,
-                                    emin=emin, emax=emax, mmax=9, tol=1e-10 * anorm,
+            lam, X, res = sm.polfed(dm, 36, math.ceil(8 / h), s=s, smin=smin, smax=smax,
+                                    emin=emin, emax=emax, mmax=mmax, tol=1e-10 * anorm,
                                     precision=prec, ctx=ctx)
             out[prec] = (np.sort(lam), float(res.max() / anorm))
     finally:
@@ -39,16 +40,24 @@ def _job(sm, ctx, A, Abig, Y):
     return out
 
 
-def test_two_contexts_two_threads(ctx):
+@pytest.mark.parametrize("split", [False, True])
+def test_two_contexts_two_threads(ctx, split, monkeypatch):
+    """With split=True every FP64 basis filter step runs as two launches on
+    each context's two streams (forced tile, 32-column blocks): four streams
+    of two contexts in flight at once."""
     import sparse_matrix_algorithms_amd as sm
+    kw = {}
+    if split:
+        monkeypatch.setenv("SMALGO_FILTER_TILE", "2,8,1,1,2")
+        kw = dict(s=32, mmax=6)
     ctxs = [sm.Context(0), sm.Context(0)]
     mats = [anderson3d(12, 16.5, 31), anderson3d(12, 16.5, 32)]
     big = anderson3d(48, 16.5, 48)
     Ys = [gaussian_block(41, big.shape[0], 160), gaussian_block(42, big.shape[0], 160)]
     try:
-        seq = [_job(sm, c, A, big, Y) for c, A, Y in zip(ctxs, mats, Ys)]
+        seq = [_job(sm, c, A, big, Y, **kw) for c, A, Y in zip(ctxs, mats, Ys)]
         with ThreadPoolExecutor(2) as ex:
-            futs = [ex.submit(_job, sm, c, A, big, Y) for c, A, Y in zip(ctxs, mats, Ys)]
+            futs = [ex.submit(_job, sm, c, A, big, Y, **kw) for c, A, Y in zip(ctxs, mats, Ys)]
             par = [f.result(timeout=300) for f in futs]
     finally:
         for c in ctxs:
@@ -59,6 +68,8 @@ def test_two_contexts_two_threads(ctx):
         exact = seq[i]["exact"]
         assert len(exact) == 30
         for prec in ("fp64", "fp32"):
+            if split:  # the same forced tile on both paths: the same bits
+                assert np.array_equal(seq[i][prec][0], par[i][prec][0]), (i, prec)
             for run in (seq[i], par[i]):
                 lam, maxres = run[prec]
                 assert len(lam) == len(exact), (i, prec, len(lam))
