@@ -534,13 +534,42 @@ static std::vector<FilterTile> tile_candidates(int b, size_t elem, bool prefetch
 // after one warm step; a tile's time (ms per step, into tmin) is its fastest
 // sweep, so clock ramps and one-off stalls do not pick the tile.  Sweeps after
 // the first re-time only the candidates within 25% of the best time so far,
-// over twice the steps.
+// over twice the steps; a run-off over four times the steps decides among the
+// tiles within 3% of the best.
 // Returns the index of the fastest.  SMALGO_TUNE_LOG=1 prints every time and
 // the pick.
 template <typename StepFn>
 static size_t time_tiles(Ctx& cx, const std::vector<FilterTile>& cand, int sweeps, int reps,
                          StepFn&& step, std::vector<float>& tmin, const std::string& what) {
   Event e0, e1, e1s;
+  // ms per step of candidate i over nrep steps after one warm step.  A split
+  // tile's steps run on both streams inside one fork/join pair, as in the
+  // recurrence; their time runs from e0 on the main stream to the later of
+  // the two streams' end events, so the fork and the join (paid once per
+  // filter, not per step) stay outside it.
+  auto time_one = [&](size_t i, int nrep) {
+    const bool sp = cand[i].split == 2;
+    bool two = false;
+    {
+      SplitScope scope(cx, sp);
+      step(cand[i]);  // warm
+      SMALGO_HIP(hipEventRecord(e0, cx.stream));
+      for (int r = 0; r < nrep; ++r) step(cand[i]);
+      SMALGO_HIP(hipEventRecord(e1, cx.stream));
+      two = sp && cx.split_active;
+      if (two) SMALGO_HIP(hipEventRecord(e1s, cx.side));
+    }
+    SMALGO_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (two) {
+      float ms2 = 0.f;
+      SMALGO_HIP(hipEventSynchronize(e1s));
+      SMALGO_HIP(hipEventElapsedTime(&ms2, e0, e1s));
+      ms = std::max(ms, ms2);
+    }
+    return ms / (float)nrep;
+  };
   tmin.assign(cand.size(), 1e30f);
   for (int sweep = 0; sweep < sweeps; ++sweep) {
     // later sweeps re-time only the tiles within 25% of the best so far, with
@@ -549,36 +578,29 @@ static size_t time_tiles(Ctx& cx, const std::vector<FilterTile>& cand, int sweep
     const int nrep = sweep > 0 ? 2 * reps : reps;
     for (size_t i = 0; i < cand.size(); ++i) {
       if (sweep > 0 && tmin[i] > cut) continue;
-      // A split tile's steps run on both streams inside one fork/join pair,
-      // as in the recurrence.  Their time runs from e0 on the main stream to
-      // the later of the two streams' end events, so the fork and the join
-      // (paid once per filter, not per step) stay outside it.
-      const bool sp = cand[i].split == 2;
-      bool two = false;
-      {
-        SplitScope scope(cx, sp);
-        step(cand[i]);  // warm
-        SMALGO_HIP(hipEventRecord(e0, cx.stream));
-        for (int r = 0; r < nrep; ++r) step(cand[i]);
-        SMALGO_HIP(hipEventRecord(e1, cx.stream));
-        two = sp && cx.split_active;
-        if (two) SMALGO_HIP(hipEventRecord(e1s, cx.side));
-      }
-      SMALGO_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      SMALGO_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (two) {
-        float ms2 = 0.f;
-        SMALGO_HIP(hipEventSynchronize(e1s));
-        SMALGO_HIP(hipEventElapsedTime(&ms2, e0, e1s));
-        ms = std::max(ms, ms2);
-      }
-      tmin[i] = std::min(tmin[i], ms / (float)nrep);
+      tmin[i] = std::min(tmin[i], time_one(i, nrep));
     }
   }
-  size_t best = 0;
-  for (size_t i = 1; i < cand.size(); ++i)
-    if (tmin[i] < tmin[best]) best = i;
+  // Run-off: the (at most four) tiles within 3% of the best are timed once
+  // more over four times the steps, and the pick is the fastest of them there.
+  std::vector<size_t> close;
+  if (sweeps > 1) {
+    const float lim = 1.03f * *std::min_element(tmin.begin(), tmin.end());
+    for (size_t i = 0; i < cand.size(); ++i)
+      if (tmin[i] <= lim) close.push_back(i);
+    std::stable_sort(close.begin(), close.end(), [&](size_t a, size_t b) { return tmin[a] < tmin[b]; });
+    if (close.size() > 4) close.resize(4);
+    if (close.size() < 2) close.clear();
+    for (size_t i : close) tmin[i] = time_one(i, 4 * reps);
+  }
+  size_t best = close.empty() ? 0 : close[0];
+  if (close.empty()) {
+    for (size_t i = 1; i < cand.size(); ++i)
+      if (tmin[i] < tmin[best]) best = i;
+  } else {
+    for (size_t i : close)
+      if (tmin[i] < tmin[best]) best = i;
+  }
   const char* lg = std::getenv("SMALGO_TUNE_LOG");
   if (lg && lg[0] == '1') {
     for (size_t i = 0; i < cand.size(); ++i)
