@@ -886,7 +886,7 @@ static int chol_inv_sync(Ctx& cx, Dense& dn, OrthWork& ow, int b) {
 template <typename T>
 static int cholqr(Ctx& cx, Dense& dn, OrthWork& ow, int n, int b, const T* W, int ldw, T* Qout,
                   int ldq, T* tmp, int ldt, double rtol, std::vector<double>* Rout,
-                  bool colscale = false) {
+                  bool colscale = false, bool one_pass_ok = false) {
   const double u = std::is_same<T, float>::value ? 0x1p-24 : 0x1p-53;
   ow.G.ensure((size_t)b * b);
   ow.Rinv.ensure((size_t)b * b);
@@ -936,6 +936,19 @@ static int cholqr(Ctx& cx, Dense& dn, OrthWork& ow, int n, int b, const T* W, in
     int info = chol_inv_sync(cx, dn, ow, b);
     double tr = 0.0;
     for (int i = 0; i < b; ++i) tr += std::max(0.0, ow.hG[(size_t)i * b + i]);
+    if (one_pass_ok && colscale && pass == 0 && info == 0) {
+      // Equilibrated columns: G = I + E.  With ||E||_F <= 0.1, cond(G) <= 1.1/0.9,
+      // so the first pass already leaves Q orthonormal to a few units of
+      // rounding (the error grows with cond(W)^2 = cond(G)); the second pass
+      // is skipped.
+      double e2 = 0.0;
+      for (int j = 0; j < b; ++j)
+        for (int i = 0; i < b; ++i) {
+          const double g = ow.hG[(size_t)j * b + i] - (i == j ? 1.0 : 0.0);
+          e2 += g * g;
+        }
+      if (e2 <= 0.01) passes = 1;
+    }
     double shift = 11.0 * ((double)n * b + (double)b * (b + 1)) * u * tr;
     int tries = 0;
     while (info != 0) {
@@ -1183,7 +1196,7 @@ static int refine_rr(Ctx& cx, Dense& dn, const Op& op, int n, int k, const doubl
   w.Q.ensure((size_t)n * ld);
   w.tmp.ensure((size_t)n * ld);
   int kq = cholqr<double>(cx, dn, w.ow, n, k, X, ldx, w.Q.p, ld, w.tmp.p, ld, rtol, nullptr,
-                          true);
+                          true, true);
   if (kq == 0) {
     lam.clear();
     res.clear();
